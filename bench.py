@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Throughput of the per-tick agent step on one MI355X.
+
+Workload: plain SIR on Barabasi-Albert(10M, m=5) (power-law degrees, mean 10),
+int64 row_ptr / int32 col_idx, degree-sorted rows, int8 states, beta = 0.03,
+gamma = 0.1, 0.1% initially infected, 200 ticks.  One step = one 200-tick
+simulation from the seeded initial state (state reset + hipGraph-captured
+tick loop).  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line: agent-ticks/s (whole job), the roofline of the tick
+kernel (algorithmic bytes from the kernel's own work counters over HIP-event
+launch time) and the CPU oracle timed on this host's cores on windows of the
+same run (each window also checked bit-exact against the GPU).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def host_threads():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def read_traffic(path):
+    """HBM bytes per tick-kernel launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    recorded as 'traffic_bytes_per_launch: <float>' in a committed profile summary."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--m", type=int, default=5)
+    ap.add_argument("--ticks", type=int, default=200)
+    ap.add_argument("--heavy", type=int, default=64, help="degree from which a row gets a whole wavefront")
+    ap.add_argument("--apl", type=int, default=1, help="agents per lane of the other rows")
+    ap.add_argument("--grid", type=int, default=0, help="blocks of the persistent grid (0: from occupancy)")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-oracle sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-account", action="store_true", help="skip the work-counter pass")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "tick_kernel_ba10m_traffic.txt"))
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    from sagesim_amd import Engine, Replica, workloads
+
+    t0 = time.perf_counter()
+    w = workloads.ba_sir(n=args.n, m=args.m, ticks=args.ticks)
+    t_gen = time.perf_counter() - t0
+    g = w.graph
+    log(f"graph BA({g.n}, {args.m}): nnz {g.nnz}, max degree {int(g.degrees().max())}, {t_gen:.1f}s")
+
+    eng = Engine(0)
+    eng.set_tiling(args.heavy, args.apl, args.grid, True)
+    t0 = time.perf_counter()
+    rep = Replica(eng, g, w.state, w.model, w.seed)
+    eng.sync()
+    t_upload = time.perf_counter() - t0
+    n_tiles, grid = eng.launch_shape()
+    log(f"upload {t_upload:.2f}s, tiles {n_tiles}, grid {grid}")
+
+    def step():
+        rep.reset()
+        eng.simulate(args.ticks)
+
+    for _ in range(args.warmup):
+        step()
+    eng.sync()
+
+    # timed region: K steps; HIP events on the engine's stream around each tick loop
+    kern_ms = 0.0
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rep.reset()
+        eng.timer_start()
+        eng.simulate(args.ticks)
+        kern_ms += eng.timer_stop()
+    eng.sync()
+    wall = time.perf_counter() - t0
+    agent_ticks = g.n * args.ticks * args.steps
+    value = agent_ticks / wall
+    launch_us = kern_ms * 1e3 / (args.steps * args.ticks)
+    final = rep.states()
+    counts = np.bincount(final, minlength=3)[:3].tolist()
+    log(f"{args.steps} steps in {wall:.3f}s: {value:.4g} agent-ticks/s, "
+        f"tick launch {launch_us:.1f} us, final S/I/R {counts}")
+
+    # algorithmic bytes of the implemented kernel, per tick, from its own counters
+    roofline = None
+    if not args.no_account:
+        eng.set_counting(True)
+        rep.reset()
+        eng.read_counters()
+        alg = 0.0
+        n_words = (g.n + 63) // 64
+        for _ in range(args.ticks):
+            eng.simulate(1)
+            s_agents, draws, scans, edges = (int(x) for x in eng.read_counters())
+            alg += 2 * g.n + 8 * n_words + 8 * s_agents + 4 * draws + 5 * edges
+        eng.set_counting(False)
+        per_launch = alg / args.ticks
+        achieved = per_launch / (launch_us * 1e-6) / 1e9
+        traffic = read_traffic(args.traffic)
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": None if traffic is None else round(traffic, 1),
+                    "alg_bytes_per_launch": round(per_launch, 1), "launch_us": round(launch_us, 2),
+                    "kernel": "tick_kernel<1>"}
+        log(f"algorithmic {per_launch / 1e6:.2f} MB/launch -> {achieved:.1f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = host_threads()
+        om = O.sir_model(w.beta, w.gamma, w.seed)
+        ids = np.arange(g.n, dtype=np.int64)
+        # windows spread over the run, each started from the GPU's bit-exact state
+        starts = [0, args.ticks // 4, args.ticks // 2, 3 * args.ticks // 4]
+        win = max(1, min(10, args.ticks // 8))
+        cpu_time, cpu_ticks, checked = 0.0, 0, True
+        budget = args.cpu_seconds
+        for s0 in starts:
+            rep.reset()
+            eng.simulate(s0)
+            st0 = rep.states()
+            eng.simulate(win)
+            want = rep.states()
+            c0 = time.perf_counter()
+            got, _ = O.sir_run(om, g.row_ptr, g.col_idx, ids, st0, win, tick0=s0, nthreads=threads)
+            dt = time.perf_counter() - c0
+            cpu_time += dt
+            cpu_ticks += win
+            checked &= bool(np.array_equal(got, want))
+            log(f"cpu window ticks [{s0}, {s0 + win}): {dt:.2f}s, match {np.array_equal(got, want)}")
+            if cpu_time > budget:
+                break
+        cpu = {"value": g.n * cpu_ticks / cpu_time, "unit": "agent-ticks/s", "cores": threads,
+               "kind": "port",
+               "sample": f"CPU oracle (C/OpenMP, {threads} threads, {cpu_model()}) on {cpu_ticks} ticks of the "
+                         f"same BA({g.n}, {args.m}) run: {win}-tick windows starting at ticks "
+                         f"{starts[:cpu_ticks // win]} from the GPU state; windows bit-exact vs GPU: {checked}"}
+
+    out = {
+        "metric": "agent-ticks/s (plain SIR, BA 10M agents, 200 ticks)",
+        "value": value,
+        "unit": "agent-ticks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8",
+        "data": "synthetic (seeded Barabasi-Albert graph and initial infections)",
+        "config": {"workload": f"plain SIR, BA(n={g.n}, m={args.m}), beta={w.beta}, gamma={w.gamma}, "
+                               f"{args.ticks} ticks, degree-sorted CSR (int64 row_ptr, int32 col_idx)",
+                   "agents": g.n, "ticks": args.ticks, "nnz": g.nnz,
+                   "tiling": {"heavy_degree": args.heavy, "agents_per_lane": args.apl, "grid": grid,
+                              "tiles": n_tiles},
+                   "graph_gen_s": round(t_gen, 2), "upload_s": round(t_upload, 2),
+                   "final_SIR": counts},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,172 @@
+/*
+ * libsagestep: C ABI of the MI355X (gfx950) per-tick agent step.
+ *
+ * The reference has no native library: its hot path is Python + CuPy JIT
+ * (sagesim/model.py, sagesim/gpu_kernels.py).  Each entry point below names
+ * the reference code whose job it takes over.  Paths are relative to the
+ * reference repository root.
+ *
+ * Conventions
+ *   - Every function returns int: SS_OK (0) or a negative SS_ERR_* code; the
+ *     message is available from ss_last_error(ctx) (ctx may be NULL for
+ *     errors raised before a context exists).
+ *   - Host pointers are borrowed for the duration of the call.  The context
+ *     owns all device memory.
+ *   - A context is single-threaded: the caller serialises calls on it.
+ *     Several contexts may coexist (each has its own HIP stream).
+ *   - Work is ordered on the context's stream.  ss_simulate returns once the
+ *     ticks are queued; ss_sync (and every download) waits for them.
+ *   - Nothing throws across the ABI.
+ */
+#ifndef SAGESTEP_H
+#define SAGESTEP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SS_OK 0
+#define SS_ERR_INVALID (-1)   /* bad argument or shape */
+#define SS_ERR_HIP (-2)       /* HIP runtime error */
+#define SS_ERR_OOM (-3)       /* device allocation failed */
+#define SS_ERR_RANGE (-4)     /* index or id outside its range */
+#define SS_ERR_STATE (-5)     /* call not valid in the current state */
+#define SS_ERR_RTC (-6)       /* hipRTC compile or load error */
+#define SS_ERR_FULL (-7)      /* hash map or budget exhausted */
+
+#define SS_HISTORY_WIDTH 32
+
+typedef struct ss_ctx ss_ctx;
+
+/* Compartment model of one replica.  An agent in s_code with k neighbours
+ * in inf_code moves to s_target when u < 1 - (1-beta)^k; an agent in state s
+ * moves to trans_next[s] when u < trans_p[s].  u = m / 2^23 where m is the 23
+ * low bits of word 0 of Philox-4x32-10(key = seed, counter = (tick, agent
+ * id)); the reference's conversion (math_utils.py:189).  The power is formed
+ * by repeated multiplication in double: q_0 = 1, q_k = q_{k-1} * (1 - beta).
+ * SIR: {S=0, I=1, R=2}; SEIR: {S=0, E=1, I=2, R=3}.  States are int8. */
+typedef struct ss_model {
+    int32_t n_states;
+    int32_t s_code;
+    int32_t inf_code;
+    int32_t s_target;
+    double beta;
+    double trans_p[4];
+    int32_t trans_next[4];
+} ss_model;
+
+/* One replica (an independent simulation).  Slots [0, n_local) are stepped;
+ * slots [n_local, n_local + n_ghost) are ghosts (model.py:1069-1073 puts
+ * ghost slots after the local ones) whose rows are empty.  row_ptr has
+ * n_local + 1 entries; col_idx values are slots in [0, n_local + n_ghost).
+ * rng_key_id[i] is the id the RNG of local slot i is keyed on (the original
+ * agent id when the caller relabels rows, model.py:1089-1096 keys the RNG on
+ * logical ids for the same reason). */
+typedef struct ss_replica_spec {
+    int64_t n_local;
+    int64_t n_ghost;
+    int64_t nnz;
+    const int64_t *row_ptr;
+    const int32_t *col_idx;
+    const int64_t *rng_key_id;
+    const int8_t *state;          /* n_local + n_ghost initial states */
+    const int32_t *history;       /* n_local * SS_HISTORY_WIDTH, -1 padded, or NULL */
+    ss_model model;
+    uint64_t seed;
+    int64_t tick0;                /* tick of the first simulated tick */
+    int64_t tick_limit;           /* the replica stops at this tick; <= 0: never */
+} ss_replica_spec;
+
+/* ---- context: replaces GPUBufferManager construction (gpu_kernels.py:263-304)
+ *      and the device setup of Model.setup (model.py:682-886) ---- */
+int ss_create(int device, ss_ctx **out);
+int ss_destroy(ss_ctx *ctx);
+const char *ss_last_error(const ss_ctx *ctx);
+int ss_sync(ss_ctx *ctx);
+/* compute units, device memory, and the bytes the context holds */
+int ss_device_info(ss_ctx *ctx, int32_t *num_cus, int64_t *total_mem, int64_t *ctx_bytes);
+
+/* ---- replicas: replace _build_gpu_buffers (model.py:1020-1274),
+ *      GPUBufferManager.allocate_property_tensors / allocate_write_buffers /
+ *      allocate_csr (gpu_kernels.py:306-359) ---- */
+int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *spec, int32_t *rep);
+int ss_replica_destroy(ss_ctx *ctx, int32_t rep);
+/* device bytes a replica of this shape takes (serving admission) */
+int64_t ss_replica_footprint(int64_t n_local, int64_t n_ghost, int64_t nnz, int32_t with_history, int64_t max_deg);
+/* back to the state, history and tick given at creation (Model.reset, model.py:944-958) */
+int ss_replica_reset(ss_ctx *ctx, int32_t rep);
+int ss_replica_tick(ss_ctx *ctx, int32_t rep, int64_t *tick);
+int ss_replica_set_tick(ss_ctx *ctx, int32_t rep, int64_t tick, int64_t tick_limit);
+int ss_replica_bytes(ss_ctx *ctx, int32_t rep, int64_t *bytes);
+int ss_set_model(ss_ctx *ctx, int32_t rep, const ss_model *model);
+/* Model.set_seed (model.py:652) */
+int ss_set_seed(ss_ctx *ctx, int32_t rep, uint64_t seed);
+
+/* ---- state readback / upload: replace get_agent_property_value's slice
+ *      .get() (model.py:422-450) and set_agent_property_value ---- */
+int ss_state_download(ss_ctx *ctx, int32_t rep, int8_t *out, int64_t n);
+int ss_state_upload(ss_ctx *ctx, int32_t rep, const int8_t *in, int64_t n);
+int ss_history_download(ss_ctx *ctx, int32_t rep, int32_t *out, int64_t n_rows);
+
+/* ---- run: replaces Model.simulate -> worker_coroutine -> stepfunc launch
+ *      (model.py:961-998, :1314-1640, kernel template :2421-2454) ----
+ * Advances every live replica by `ticks` ticks (each stops at its own
+ * tick_limit).  Tick loops are captured once per tick count in a hipGraph. */
+int ss_simulate(ss_ctx *ctx, int64_t ticks);
+/* Launch shape: heavy_degree = rows with at least this degree at the front
+ * of a replica get one wavefront each; agents_per_lane in {1,2,4} for the
+ * other rows; grid_blocks = 0 for a persistent grid sized from occupancy;
+ * use_graph = 0 launches tick by tick. */
+int ss_set_tiling(ss_ctx *ctx, int32_t heavy_degree, int32_t agents_per_lane, int32_t grid_blocks,
+                  int32_t use_graph);
+/* tiles and grid of the current launch shape (after ss_simulate built them) */
+int ss_launch_shape(ss_ctx *ctx, int32_t *n_tiles, int32_t *grid);
+
+/* ---- ghosts of one partition: replace GPUHashMap (gpu_kernels.py:91-260,
+ *      which the reference builds but never queries) and the pack/unpack of
+ *      CommunicationManager (gpu_kernels.py:852-909, :984-1066) ---- */
+/* (Re)build the id -> slot map; capacity = max(64, ceil(n / load_factor)). */
+int ss_map_build(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int32_t *slots, int64_t n,
+                 double load_factor);
+int ss_map_insert(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int32_t *slots, int64_t n);
+int ss_map_remove(ss_ctx *ctx, int32_t rep, const int64_t *ids, int64_t n, int64_t *n_removed);
+/* out[i] = slot of ids[i] or -1 (GPUHashMap.lookup_cpu, gpu_kernels.py:208-215) */
+int ss_map_lookup(ss_ctx *ctx, int32_t rep, const int64_t *ids, int32_t *out, int64_t n, int64_t *n_missing);
+/* Fix the record order of the halo buffer: record h refreshes the ghost with
+ * id ids[h] (build_communication_maps, gpu_kernels.py:521-850, resolves the
+ * recv index lists once). */
+int ss_halo_bind(ss_ctx *ctx, int32_t rep, const int64_t *ids, int64_t n, int64_t *n_missing);
+/* Refresh ghosts from one halo buffer in bound order (vals[n_bound]);
+ * both state buffers are written (gpu_kernels.py:1030-1036). */
+int ss_halo_unpack(ss_ctx *ctx, int32_t rep, const int8_t *vals, int64_t n);
+/* Refresh ghosts from (id, state) records in any order. */
+int ss_halo_unpack_ids(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int8_t *vals, int64_t n,
+                       int64_t *n_missing);
+/* out[h] = current state of slot slots[h] (the send side, gpu_kernels.py:852-909) */
+int ss_halo_pack(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n, int8_t *out);
+/* Run `ticks` ticks; before tick j the ghosts are refreshed from
+ * vals[j * n_bound ...] (bound order), staged through pinned memory and
+ * copied on a second stream while the previous tick computes. */
+int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals);
+
+/* ---- measurement: replaces the verbose_timing dict of worker_coroutine
+ *      (model.py:1351-1640) with device-side events and counters ---- */
+/* Count the work of every tick launch: out[0] susceptible agents visited,
+ * out[1] RNG draws, out[2] neighbour scans started, out[3] CSR entries read. */
+int ss_set_counting(ss_ctx *ctx, int32_t on);
+/* read and clear the work counters (int64 out[4]) */
+int ss_read_counters(ss_ctx *ctx, int64_t *out);
+/* census of the live state: out[0..3] agents per state, out[4] sum of the
+ * degrees of susceptible agents (int64 out[5]) */
+int ss_census(ss_ctx *ctx, int32_t rep, int64_t *out);
+/* HIP events on the context's stream around queued work */
+int ss_timer_start(ss_ctx *ctx);
+int ss_timer_stop(ss_ctx *ctx, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SAGESTEP_H */

@@ -1,0 +1,892 @@
+// libsagestep runtime: contexts, replicas, tick loops, ghosts (C ABI of
+// include/sagestep.h).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/sagestep.h"
+#include "ss_runtime.h"
+
+static thread_local std::string g_err;
+
+#define SS_FAIL(ctx, code, ...)                                        \
+    do {                                                               \
+        char _b[512];                                                  \
+        std::snprintf(_b, sizeof _b, __VA_ARGS__);                     \
+        if (ctx) static_cast<ss_ctx *>(ctx)->err = _b; else g_err = _b; \
+        return code;                                                   \
+    } while (0)
+
+#define SS_HIP(ctx, expr)                                                               \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess)                                                           \
+            SS_FAIL(ctx, _e == hipErrorOutOfMemory ? SS_ERR_OOM : SS_ERR_HIP, "%s: %s (%s:%d)", \
+                    #expr, hipGetErrorString(_e), __FILE__, __LINE__);                  \
+    } while (0)
+
+namespace {
+
+template <class T>
+hipError_t dalloc(ss_ctx *ctx, ss_replica *r, T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    size_t b = sizeof(T) * count;
+    hipError_t e = hipMalloc((void **)p, b);
+    if (e == hipSuccess) {
+        if (r) r->bytes += b;
+        ctx->bytes += b;
+    }
+    return e;
+}
+
+void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+// inf_T[k] = ceil(p_k * 2^23), p_k = 1 - q_k, q_k = q_{k-1} * (1 - beta),
+// for k = 0 .. min(max_deg, first k with p_k * 2^23 == 2^23).
+std::vector<uint32_t> threshold_table(double beta, int64_t max_deg) {
+    std::vector<uint32_t> T;
+    T.push_back(0u);
+    double q = 1.0;
+    const double b = 1.0 - beta;
+    for (int64_t k = 1; k <= max_deg; ++k) {
+        q = q * b;
+        const double p = 1.0 - q;
+        const double x = p * 8388608.0;
+        const uint32_t v = (uint32_t)std::ceil(x);
+        T.push_back(v);
+        if (v >= 8388608u) break;
+    }
+    return T;
+}
+
+uint32_t prob_threshold(double p) {
+    if (!(p > 0.0)) return 0u;
+    if (p >= 1.0) return 8388608u;
+    return (uint32_t)std::ceil(p * 8388608.0);
+}
+
+bool valid_rep(ss_ctx *ctx, int32_t rep) {
+    return ctx && rep >= 0 && rep < (int32_t)ctx->reps.size() && ctx->reps[rep].live;
+}
+
+int check_model(ss_ctx *ctx, const ss_model *m) {
+    if (!m) SS_FAIL(ctx, SS_ERR_INVALID, "model is NULL");
+    if (m->n_states < 1 || m->n_states > 4) SS_FAIL(ctx, SS_ERR_INVALID, "n_states must be 1..4");
+    auto in = [&](int32_t s) { return s >= 0 && s < m->n_states; };
+    if (!in(m->s_code) || !in(m->inf_code) || !in(m->s_target))
+        SS_FAIL(ctx, SS_ERR_INVALID, "model state codes out of range");
+    if (!(m->beta >= 0.0 && m->beta <= 1.0)) SS_FAIL(ctx, SS_ERR_INVALID, "beta must be in [0, 1]");
+    for (int s = 0; s < 4; ++s) {
+        if (!(m->trans_p[s] >= 0.0 && m->trans_p[s] <= 1.0))
+            SS_FAIL(ctx, SS_ERR_INVALID, "trans_p[%d] must be in [0, 1]", s);
+        if (s < m->n_states && m->trans_p[s] > 0.0 && !in(m->trans_next[s]))
+            SS_FAIL(ctx, SS_ERR_INVALID, "trans_next[%d] out of range", s);
+    }
+    return SS_OK;
+}
+
+int apply_model(ss_ctx *ctx, ss_replica &r, const ss_model *m) {
+    int rc = check_model(ctx, m);
+    if (rc) return rc;
+    std::vector<uint32_t> T = threshold_table(m->beta, r.max_deg);
+    dfree(r.inf_T);
+    r.inf_T = nullptr;
+    SS_HIP(ctx, dalloc(ctx, &r, &r.inf_T, T.size()));
+    SS_HIP(ctx, hipMemcpy(r.inf_T, T.data(), T.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    r.dev.inf_T = r.inf_T;
+    r.dev.inf_T_last = (int32_t)T.size() - 1;
+    for (int s = 0; s < 4; ++s) {
+        r.dev.trans_T[s] = s < m->n_states && s != m->s_code ? prob_threshold(m->trans_p[s]) : 0u;
+        r.dev.trans_next[s] = (int8_t)m->trans_next[s];
+    }
+    r.dev.s_code = (int8_t)m->s_code;
+    r.dev.inf_code = (int8_t)m->inf_code;
+    r.dev.s_target = (int8_t)m->s_target;
+    r.dev.n_states = (int8_t)m->n_states;
+    return SS_OK;
+}
+
+void free_replica(ss_replica &r) {
+    dfree(r.row_ptr); dfree(r.col_idx); dfree(r.rng_lo); dfree(r.rng_hi);
+    dfree(r.state[0]); dfree(r.state[1]); dfree(r.init_state);
+    dfree(r.ibits[0]); dfree(r.ibits[1]);
+    dfree(r.history); dfree(r.init_history); dfree(r.inf_T);
+    dfree(r.map_keys); dfree(r.map_vals); dfree(r.halo_slots);
+    dfree(r.halo_dev[0]); dfree(r.halo_dev[1]);
+    r = ss_replica();
+}
+
+// rebuild the infectious bitmap of buffer `par` over words [w0, w1)
+int rebuild_bits(ss_ctx *ctx, ss_replica &r, int par, int64_t w0, int64_t w1) {
+    SS_HIP(ctx, ss_launch_bits(r.state[par], r.ibits[par], w0, w1, r.n_total, r.dev.inf_code, ctx->stream));
+    return SS_OK;
+}
+
+void drop_graphs(ss_ctx *ctx) {
+    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
+}
+
+int64_t heavy_prefix(const ss_replica &r, int32_t thr) {
+    for (size_t q = 0; q < r.heavy_prefix_deg.size(); ++q)
+        if (r.heavy_prefix_deg[q] == thr) return r.heavy_prefix_len[q];
+    return -1;
+}
+
+int build_tiles(ss_ctx *ctx) {
+    std::vector<ss_tile> heavy, light;
+    const int64_t cap = (int64_t)SS_BLOCK * ctx->apl;
+    for (int32_t ri = 0; ri < (int32_t)ctx->reps.size(); ++ri) {
+        ss_replica &r = ctx->reps[ri];
+        if (!r.live || r.n_total == 0) continue;
+        int64_t H = heavy_prefix(r, ctx->heavy_deg);
+        if (H < 0) SS_FAIL(ctx, SS_ERR_INVALID, "heavy_degree %d has no prefix table", ctx->heavy_deg);
+        int64_t H64 = std::min<int64_t>((H + 63) / 64 * 64, r.n_total);
+        for (int64_t b = 0; b < H64; b += 64)
+            heavy.push_back({ri, SS_TILE_HEAVY, b, std::min<int64_t>(b + 64, r.n_total), 0});
+        for (int64_t b = H64; b < r.n_total; b += cap)
+            light.push_back({ri, SS_TILE_LIGHT, b, std::min<int64_t>(b + cap, r.n_total), 0});
+    }
+    std::vector<ss_tile> all;
+    all.reserve(heavy.size() + light.size());
+    all.insert(all.end(), heavy.begin(), heavy.end());
+    all.insert(all.end(), light.begin(), light.end());
+    if ((int64_t)all.size() > INT32_MAX) SS_FAIL(ctx, SS_ERR_RANGE, "too many tiles");
+    if ((int64_t)all.size() > ctx->tiles_cap) {
+        dfree(ctx->d_tiles);
+        ctx->d_tiles = nullptr;
+        int64_t nc = std::max<int64_t>((int64_t)all.size() * 5 / 4, 1024);
+        SS_HIP(ctx, dalloc(ctx, nullptr, &ctx->d_tiles, (size_t)nc));
+        ctx->tiles_cap = nc;
+    }
+    if (!all.empty())
+        SS_HIP(ctx, hipMemcpyAsync(ctx->d_tiles, all.data(), all.size() * sizeof(ss_tile), hipMemcpyHostToDevice,
+                                   ctx->stream));
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->n_tiles = (int)all.size();
+    int bpc = 0;
+    SS_HIP(ctx, ss_occupancy_tick(ctx->apl, &bpc));
+    if (bpc < 1) bpc = 1;
+    int g = ctx->grid_blocks > 0 ? ctx->grid_blocks : ctx->num_cus * bpc;
+    ctx->grid = std::max(1, std::min(g, std::max(ctx->n_tiles, 1)));
+    drop_graphs(ctx);
+    ctx->tiles_dirty = false;
+    return SS_OK;
+}
+
+int ensure_rep_table(ss_ctx *ctx) {
+    int need = (int)ctx->reps.size();
+    if (need <= ctx->reps_cap) return SS_OK;
+    int nc = std::max(need * 2, 16);
+    ss_rep_dev *d = nullptr, *h = nullptr;
+    SS_HIP(ctx, dalloc(ctx, nullptr, &d, (size_t)nc));
+    SS_HIP(ctx, hipHostMalloc((void **)&h, sizeof(ss_rep_dev) * nc, hipHostMallocDefault));
+    std::memset(h, 0, sizeof(ss_rep_dev) * nc);
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    dfree(ctx->d_reps);
+    if (ctx->h_reps) (void)hipHostFree(ctx->h_reps);
+    ctx->d_reps = d;
+    ctx->h_reps = h;
+    ctx->reps_cap = nc;
+    drop_graphs(ctx);  // captured launches hold the old table pointer
+    return SS_OK;
+}
+
+// copy descriptors with tick window [tick, tick + ticks) clipped to each limit
+int push_descriptors(ss_ctx *ctx, int64_t ticks) {
+    int rc = ensure_rep_table(ctx);
+    if (rc) return rc;
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));  // the mirror may still be in flight
+    for (size_t i = 0; i < ctx->reps.size(); ++i) {
+        ss_replica &r = ctx->reps[i];
+        ss_rep_dev d = r.dev;
+        d.counters = ctx->counting ? ctx->d_work : nullptr;
+        if (r.live) {
+            d.tick0 = r.tick;
+            int64_t end = ticks > INT64_MAX - r.tick ? INT64_MAX : r.tick + ticks;
+            d.tick_end = std::min(end, r.tick_limit);
+        } else {
+            d.tick0 = 0;
+            d.tick_end = 0;
+        }
+        ctx->h_reps[i] = d;
+    }
+    SS_HIP(ctx, hipMemcpyAsync(ctx->d_reps, ctx->h_reps, sizeof(ss_rep_dev) * ctx->reps.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    return SS_OK;
+}
+
+// after a loop of `ticks` launches every live replica sits at
+// min(tick + ticks, tick_limit), or where it was if already past its limit
+void advance_ticks(ss_ctx *ctx, int64_t ticks) {
+    for (auto &r : ctx->reps)
+        if (r.live) {
+            int64_t end = ticks > INT64_MAX - r.tick ? INT64_MAX : r.tick + ticks;
+            end = std::min(end, r.tick_limit);
+            r.tick = std::max(r.tick, end);
+        }
+}
+
+int launch_loop(ss_ctx *ctx, int64_t ticks) {
+    if (ticks > INT32_MAX) SS_FAIL(ctx, SS_ERR_RANGE, "ticks per call must fit int32");
+    if (!ctx->use_graph) {
+        for (int64_t k = 0; k < ticks; ++k)
+            SS_HIP(ctx, ss_launch_tick(ctx->d_reps, ctx->d_tiles, ctx->n_tiles, (int)k, ctx->apl, ctx->grid,
+                                       ctx->stream));
+        return SS_OK;
+    }
+    auto it = ctx->graphs.find(ticks);
+    if (it == ctx->graphs.end()) {
+        hipGraph_t g = nullptr;
+        SS_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t le = hipSuccess;
+        for (int64_t k = 0; k < ticks && le == hipSuccess; ++k)
+            le = ss_launch_tick(ctx->d_reps, ctx->d_tiles, ctx->n_tiles, (int)k, ctx->apl, ctx->grid, ctx->stream);
+        hipError_t ee = hipStreamEndCapture(ctx->stream, &g);
+        SS_HIP(ctx, le);
+        SS_HIP(ctx, ee);
+        hipGraphExec_t ex = nullptr;
+        hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        SS_HIP(ctx, ie);
+        it = ctx->graphs.emplace(ticks, ex).first;
+    }
+    SS_HIP(ctx, hipGraphLaunch(it->second, ctx->stream));
+    return SS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ss_last_error(const ss_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int ss_create(int device, ss_ctx **out) {
+    if (!out) SS_FAIL(nullptr, SS_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) SS_FAIL(nullptr, SS_ERR_HIP, "no HIP device: %s", hipGetErrorString(e));
+    if (device < 0 || device >= ndev) SS_FAIL(nullptr, SS_ERR_INVALID, "device %d out of range", device);
+    auto *ctx = new ss_ctx();
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    if ((e = hipSetDevice(device)) != hipSuccess || (e = hipGetDeviceProperties(&prop, device)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&ctx->halo_ev[0], hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&ctx->halo_ev[1], hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreate(&ctx->t_start)) != hipSuccess || (e = hipEventCreate(&ctx->t_stop)) != hipSuccess ||
+        (e = hipMalloc((void **)&ctx->d_work, 4 * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMalloc((void **)&ctx->d_stats, 8 * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMemset(ctx->d_work, 0, 4 * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMalloc((void **)&ctx->d_counters, 64 * sizeof(int32_t))) != hipSuccess) {
+        delete ctx;
+        SS_FAIL(nullptr, SS_ERR_HIP, "context setup: %s", hipGetErrorString(e));
+    }
+    ctx->num_cus = prop.multiProcessorCount;
+    ctx->total_mem = (int64_t)prop.totalGlobalMem;
+    *out = ctx;
+    return SS_OK;
+}
+
+int ss_destroy(ss_ctx *ctx) {
+    if (!ctx) return SS_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->copy_stream);
+    drop_graphs(ctx);
+    for (auto &r : ctx->reps) free_replica(r);
+    dfree(ctx->d_reps);
+    dfree(ctx->d_tiles);
+    dfree(ctx->d_counters);
+    dfree(ctx->d_work);
+    dfree(ctx->d_stats);
+    if (ctx->t_start) (void)hipEventDestroy(ctx->t_start);
+    if (ctx->t_stop) (void)hipEventDestroy(ctx->t_stop);
+    if (ctx->h_reps) (void)hipHostFree(ctx->h_reps);
+    for (int i = 0; i < 2; ++i) {
+        if (ctx->h_halo_pin[i]) (void)hipHostFree(ctx->h_halo_pin[i]);
+        if (ctx->halo_ev[i]) (void)hipEventDestroy(ctx->halo_ev[i]);
+    }
+    (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->copy_stream);
+    delete ctx;
+    return SS_OK;
+}
+
+int ss_sync(ss_ctx *ctx) {
+    if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SS_OK;
+}
+
+int ss_device_info(ss_ctx *ctx, int32_t *num_cus, int64_t *total_mem, int64_t *ctx_bytes) {
+    if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
+    if (num_cus) *num_cus = ctx->num_cus;
+    if (total_mem) *total_mem = ctx->total_mem;
+    if (ctx_bytes) *ctx_bytes = (int64_t)ctx->bytes;
+    return SS_OK;
+}
+
+int64_t ss_replica_footprint(int64_t n_local, int64_t n_ghost, int64_t nnz, int32_t with_history, int64_t max_deg) {
+    int64_t n_total = n_local + n_ghost;
+    int64_t words = (n_total + 63) / 64;
+    int64_t b = 8 * (n_total + 1) + 4 * nnz + 4 * n_local + 3 * n_total + 2 * 8 * words +
+                4 * (std::min<int64_t>(max_deg, 1 << 24) + 1);
+    if (with_history) b += 2 * 4 * n_local * SS_HISTORY_WIDTH;
+    return b;
+}
+
+int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) {
+    if (!ctx || !sp || !rep_out) SS_FAIL(ctx, SS_ERR_INVALID, "NULL argument");
+    (void)hipSetDevice(ctx->device);
+    const int64_t nl = sp->n_local, ng = sp->n_ghost, nnz = sp->nnz, nt = nl + ng;
+    if (nl < 0 || ng < 0 || nnz < 0 || nt > INT32_MAX) SS_FAIL(ctx, SS_ERR_RANGE, "slot counts out of range");
+    if ((nl > 0 && (!sp->row_ptr || !sp->rng_key_id)) || (nt > 0 && !sp->state) || (nnz > 0 && !sp->col_idx))
+        SS_FAIL(ctx, SS_ERR_INVALID, "missing arrays");
+    // shape checks on the host before anything reaches a kernel
+    int64_t max_deg = 0;
+    if (nl > 0) {
+        if (sp->row_ptr[0] != 0 || sp->row_ptr[nl] != nnz) SS_FAIL(ctx, SS_ERR_INVALID, "row_ptr[0] must be 0 and row_ptr[n_local] == nnz");
+        for (int64_t i = 0; i < nl; ++i) {
+            int64_t d = sp->row_ptr[i + 1] - sp->row_ptr[i];
+            if (d < 0) SS_FAIL(ctx, SS_ERR_INVALID, "row_ptr decreases at row %lld", (long long)i);
+            max_deg = std::max(max_deg, d);
+        }
+    } else if (nnz != 0) {
+        SS_FAIL(ctx, SS_ERR_INVALID, "nnz must be 0 without local rows");
+    }
+    for (int64_t e = 0; e < nnz; ++e)
+        if (sp->col_idx[e] < 0 || sp->col_idx[e] >= nt)
+            SS_FAIL(ctx, SS_ERR_RANGE, "col_idx[%lld] = %d outside [0, %lld)", (long long)e, sp->col_idx[e], (long long)nt);
+    int rc = check_model(ctx, &sp->model);
+    if (rc) return rc;
+    bool need_hi = false;
+    for (int64_t i = 0; i < nl; ++i)
+        if ((uint64_t)sp->rng_key_id[i] >> 32) { need_hi = true; break; }
+
+    ss_replica r;
+    r.n_local = nl; r.n_ghost = ng; r.n_total = nt; r.nnz = nnz;
+    r.n_words = (nt + 63) / 64;
+    r.max_deg = max_deg;
+    // heavy prefix lengths for the tiling thresholds the autotuner may pick
+    for (int32_t thr : {8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 512, 1024, 1 << 30}) {
+        int64_t H = 0;
+        while (H < nl && sp->row_ptr[H + 1] - sp->row_ptr[H] >= thr) ++H;
+        r.heavy_prefix_deg.push_back(thr);
+        r.heavy_prefix_len.push_back(H);
+    }
+    auto fail = [&](int code) {
+        ctx->bytes -= r.bytes;
+        free_replica(r);
+        return code;
+    };
+#define R_HIP(expr)                                                                  \
+    do {                                                                             \
+        hipError_t _e = (expr);                                                      \
+        if (_e != hipSuccess) {                                                      \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(_e);            \
+            return fail(_e == hipErrorOutOfMemory ? SS_ERR_OOM : SS_ERR_HIP);        \
+        }                                                                            \
+    } while (0)
+    R_HIP(dalloc(ctx, &r, &r.row_ptr, (size_t)nt + 1));
+    R_HIP(dalloc(ctx, &r, &r.col_idx, (size_t)nnz));
+    R_HIP(dalloc(ctx, &r, &r.rng_lo, (size_t)nl));
+    if (need_hi) R_HIP(dalloc(ctx, &r, &r.rng_hi, (size_t)nl));
+    for (int b = 0; b < 2; ++b) {
+        R_HIP(dalloc(ctx, &r, &r.state[b], (size_t)nt));
+        R_HIP(dalloc(ctx, &r, &r.ibits[b], (size_t)r.n_words));
+    }
+    R_HIP(dalloc(ctx, &r, &r.init_state, (size_t)nt));
+    if (sp->history) {
+        R_HIP(dalloc(ctx, &r, &r.history, (size_t)nl * SS_HISTORY_WIDTH));
+        R_HIP(dalloc(ctx, &r, &r.init_history, (size_t)nl * SS_HISTORY_WIDTH));
+    }
+    // uploads (synchronous: host arrays are borrowed only for this call)
+    {
+        std::vector<int64_t> rp((size_t)nt + 1);
+        if (nl > 0) std::memcpy(rp.data(), sp->row_ptr, sizeof(int64_t) * (nl + 1));
+        else rp[0] = 0;
+        for (int64_t i = nl + 1; i <= nt; ++i) rp[i] = nnz;  // ghost rows are empty
+        R_HIP(hipMemcpy(r.row_ptr, rp.data(), sizeof(int64_t) * rp.size(), hipMemcpyHostToDevice));
+    }
+    if (nnz) R_HIP(hipMemcpy(r.col_idx, sp->col_idx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+    {
+        std::vector<uint32_t> lo((size_t)std::max<int64_t>(nl, 1)), hi;
+        if (need_hi) hi.resize((size_t)nl);
+        for (int64_t i = 0; i < nl; ++i) {
+            lo[i] = (uint32_t)sp->rng_key_id[i];
+            if (need_hi) hi[i] = (uint32_t)((uint64_t)sp->rng_key_id[i] >> 32);
+        }
+        if (nl) R_HIP(hipMemcpy(r.rng_lo, lo.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice));
+        if (need_hi) R_HIP(hipMemcpy(r.rng_hi, hi.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice));
+    }
+    if (nt) R_HIP(hipMemcpy(r.init_state, sp->state, (size_t)nt, hipMemcpyHostToDevice));
+    if (sp->history)
+        R_HIP(hipMemcpy(r.init_history, sp->history, sizeof(int32_t) * nl * SS_HISTORY_WIDTH, hipMemcpyHostToDevice));
+    r.dev = ss_rep_dev{};
+    r.dev.row_ptr = r.row_ptr;
+    r.dev.col_idx = r.col_idx;
+    r.dev.rng_lo = r.rng_lo;
+    r.dev.rng_hi = r.rng_hi;
+    r.dev.state[0] = r.state[0];
+    r.dev.state[1] = r.state[1];
+    r.dev.ibits[0] = r.ibits[0];
+    r.dev.ibits[1] = r.ibits[1];
+    r.dev.history = r.history;
+    r.dev.seed_lo = (uint32_t)sp->seed;
+    r.dev.seed_hi = (uint32_t)(sp->seed >> 32);
+    r.dev.n_local = nl;
+    r.dev.n_total = nt;
+    rc = apply_model(ctx, r, &sp->model);
+    if (rc) return fail(rc);
+    r.tick = r.tick_init = sp->tick0;
+    r.tick_limit = sp->tick_limit > 0 ? sp->tick_limit : INT64_MAX;
+    r.live = true;
+    // slot reuse keeps replica ids small for long-running serving loops
+    int32_t id = -1;
+    for (int32_t i = 0; i < (int32_t)ctx->reps.size(); ++i)
+        if (!ctx->reps[i].live) { id = i; break; }
+    if (id < 0) { id = (int32_t)ctx->reps.size(); ctx->reps.emplace_back(); }
+    ctx->reps[id] = r;
+    ctx->tiles_dirty = true;
+    *rep_out = id;
+#undef R_HIP
+    return ss_replica_reset(ctx, id);
+}
+
+int ss_replica_destroy(ss_ctx *ctx, int32_t rep) {
+    if (!valid_rep(ctx, rep)) SS_FAIL(ctx, SS_ERR_INVALID, "no live replica %d", rep);
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SS_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));
+    ctx->bytes -= ctx->reps[rep].bytes;
+    free_replica(ctx->reps[rep]);
+    ctx->tiles_dirty = true;
+    return SS_OK;
+}
+
+int ss_replica_reset(ss_ctx *ctx, int32_t rep) {
+    if (!valid_rep(ctx, rep)) SS_FAIL(ctx, SS_ERR_INVALID, "no live replica %d", rep);
+    ss_replica &r = ctx->reps[rep];
+    r.tick = r.tick_init;
+    const int par = (int)(r.tick & 1);
+    if (r.n_total) {
+        SS_HIP(ctx, hipMemcpyAsync(r.state[par], r.init_state, (size_t)r.n_total, hipMemcpyDeviceToDevice, ctx->stream));
+        SS_HIP(ctx, hipMemcpyAsync(r.state[par ^ 1], r.init_state, (size_t)r.n_total, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    if (r.history)
+        SS_HIP(ctx, hipMemcpyAsync(r.history, r.init_history, sizeof(int32_t) * r.n_local * SS_HISTORY_WIDTH,
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+    int rc = rebuild_bits(ctx, r, par, 0, r.n_words);
+    if (rc) return rc;
+    return rebuild_bits(ctx, r, par ^ 1, 0, r.n_words);
+}
+
+int ss_replica_tick(ss_ctx *ctx, int32_t rep, int64_t *tick) {
+    if (!valid_rep(ctx, rep) || !tick) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    *tick = ctx->reps[rep].tick;
+    return SS_OK;
+}
+
+int ss_replica_set_tick(ss_ctx *ctx, int32_t rep, int64_t tick, int64_t tick_limit) {
+    if (!valid_rep(ctx, rep) || tick < 0) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or tick");
+    ss_replica &r = ctx->reps[rep];
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if ((tick & 1) != (r.tick & 1) && r.n_total) {
+        // keep "tick t reads state[t & 1]": move the live buffer
+        const int from = (int)(r.tick & 1);
+        SS_HIP(ctx, hipMemcpyAsync(r.state[from ^ 1], r.state[from], (size_t)r.n_total, hipMemcpyDeviceToDevice, ctx->stream));
+        int rc = rebuild_bits(ctx, r, from ^ 1, 0, r.n_words);
+        if (rc) return rc;
+    }
+    r.tick = tick;
+    r.tick_limit = tick_limit > 0 ? tick_limit : INT64_MAX;
+    return SS_OK;
+}
+
+int ss_replica_bytes(ss_ctx *ctx, int32_t rep, int64_t *bytes) {
+    if (!valid_rep(ctx, rep) || !bytes) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    *bytes = (int64_t)ctx->reps[rep].bytes;
+    return SS_OK;
+}
+
+int ss_set_model(ss_ctx *ctx, int32_t rep, const ss_model *m) {
+    if (!valid_rep(ctx, rep)) SS_FAIL(ctx, SS_ERR_INVALID, "no live replica %d", rep);
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ss_replica &r = ctx->reps[rep];
+    int rc = apply_model(ctx, r, m);
+    if (rc) return rc;
+    const int par = (int)(r.tick & 1);
+    return rebuild_bits(ctx, r, par, 0, r.n_words);
+}
+
+int ss_set_seed(ss_ctx *ctx, int32_t rep, uint64_t seed) {
+    if (!valid_rep(ctx, rep)) SS_FAIL(ctx, SS_ERR_INVALID, "no live replica %d", rep);
+    ctx->reps[rep].dev.seed_lo = (uint32_t)seed;
+    ctx->reps[rep].dev.seed_hi = (uint32_t)(seed >> 32);
+    return SS_OK;
+}
+
+int ss_state_download(ss_ctx *ctx, int32_t rep, int8_t *out, int64_t n) {
+    if (!valid_rep(ctx, rep) || (!out && n)) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    ss_replica &r = ctx->reps[rep];
+    if (n != r.n_total) SS_FAIL(ctx, SS_ERR_INVALID, "n = %lld, replica has %lld slots", (long long)n, (long long)r.n_total);
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (n) SS_HIP(ctx, hipMemcpy(out, r.state[r.tick & 1], (size_t)n, hipMemcpyDeviceToHost));
+    return SS_OK;
+}
+
+int ss_state_upload(ss_ctx *ctx, int32_t rep, const int8_t *in, int64_t n) {
+    if (!valid_rep(ctx, rep) || (!in && n)) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    ss_replica &r = ctx->reps[rep];
+    if (n != r.n_total) SS_FAIL(ctx, SS_ERR_INVALID, "n = %lld, replica has %lld slots", (long long)n, (long long)r.n_total);
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int par = (int)(r.tick & 1);
+    if (n) {
+        SS_HIP(ctx, hipMemcpy(r.state[par], in, (size_t)n, hipMemcpyHostToDevice));
+        SS_HIP(ctx, hipMemcpy(r.state[par ^ 1], in, (size_t)n, hipMemcpyHostToDevice));
+    }
+    int rc = rebuild_bits(ctx, r, par, 0, r.n_words);
+    if (rc) return rc;
+    return rebuild_bits(ctx, r, par ^ 1, 0, r.n_words);
+}
+
+int ss_history_download(ss_ctx *ctx, int32_t rep, int32_t *out, int64_t n_rows) {
+    if (!valid_rep(ctx, rep) || !out) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    ss_replica &r = ctx->reps[rep];
+    if (!r.history) SS_FAIL(ctx, SS_ERR_STATE, "replica has no history property");
+    if (n_rows != r.n_local) SS_FAIL(ctx, SS_ERR_INVALID, "n_rows must equal n_local");
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SS_HIP(ctx, hipMemcpy(out, r.history, sizeof(int32_t) * n_rows * SS_HISTORY_WIDTH, hipMemcpyDeviceToHost));
+    return SS_OK;
+}
+
+int ss_set_tiling(ss_ctx *ctx, int32_t heavy_degree, int32_t apl, int32_t grid_blocks, int32_t use_graph) {
+    if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
+    if (apl != 1 && apl != 2 && apl != 4) SS_FAIL(ctx, SS_ERR_INVALID, "agents_per_lane must be 1, 2 or 4");
+    static const int32_t ok[] = {8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 512, 1024, 1 << 30};
+    bool found = false;
+    for (int32_t v : ok) found |= v == heavy_degree;
+    if (!found) SS_FAIL(ctx, SS_ERR_INVALID, "heavy_degree must be one of 8,16,24,32,48,64,96,128,192,256,512,1024,2^30");
+    if (grid_blocks < 0) SS_FAIL(ctx, SS_ERR_INVALID, "grid_blocks must be >= 0");
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->heavy_deg = heavy_degree;
+    ctx->apl = apl;
+    ctx->grid_blocks = grid_blocks;
+    ctx->use_graph = use_graph ? 1 : 0;
+    ctx->tiles_dirty = true;
+    return SS_OK;
+}
+
+int ss_launch_shape(ss_ctx *ctx, int32_t *n_tiles, int32_t *grid) {
+    if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
+    if (ctx->tiles_dirty) {
+        int rc = build_tiles(ctx);
+        if (rc) return rc;
+    }
+    if (n_tiles) *n_tiles = ctx->n_tiles;
+    if (grid) *grid = ctx->grid;
+    return SS_OK;
+}
+
+int ss_simulate(ss_ctx *ctx, int64_t ticks) {
+    if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
+    if (ticks < 0) SS_FAIL(ctx, SS_ERR_INVALID, "ticks must be >= 0");
+    if (ticks == 0) return SS_OK;
+    (void)hipSetDevice(ctx->device);
+    int rc;
+    if (ctx->tiles_dirty && (rc = build_tiles(ctx))) return rc;
+    if ((rc = push_descriptors(ctx, ticks))) return rc;
+    if ((rc = launch_loop(ctx, ticks))) return rc;
+    advance_ticks(ctx, ticks);
+    return SS_OK;
+}
+
+// ---------------- ghosts ----------------
+
+int ss_map_build(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int32_t *slots, int64_t n, double lf) {
+    if (!valid_rep(ctx, rep) || (n && (!ids || !slots))) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    if (!(lf > 0.0 && lf <= 0.9)) SS_FAIL(ctx, SS_ERR_INVALID, "load_factor must be in (0, 0.9]");
+    ss_replica &r = ctx->reps[rep];
+    for (int64_t i = 0; i < n; ++i) {
+        if (ids[i] == -1 || ids[i] == -2) SS_FAIL(ctx, SS_ERR_RANGE, "ids -1 and -2 are reserved");
+        if (slots[i] < 0 || slots[i] >= r.n_total) SS_FAIL(ctx, SS_ERR_RANGE, "slot %d outside the replica", slots[i]);
+    }
+    int64_t cap = std::max<int64_t>(64, (int64_t)std::ceil((double)n / lf));
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (cap != r.map_cap) {
+        if (r.map_keys) { r.bytes -= sizeof(unsigned long long) * r.map_cap + sizeof(int32_t) * r.map_cap; }
+        dfree(r.map_keys); dfree(r.map_vals);
+        r.map_keys = nullptr; r.map_vals = nullptr;
+        SS_HIP(ctx, dalloc(ctx, &r, &r.map_keys, (size_t)cap));
+        SS_HIP(ctx, dalloc(ctx, &r, &r.map_vals, (size_t)cap));
+        r.map_cap = cap;
+    }
+    SS_HIP(ctx, ss_launch_map_clear(r.map_keys, r.map_vals, cap, ctx->stream));
+    r.map_size = 0;
+    return ss_map_insert(ctx, rep, ids, slots, n);
+}
+
+int ss_map_insert(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int32_t *slots, int64_t n) {
+    if (!valid_rep(ctx, rep) || (n && (!ids || !slots))) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    ss_replica &r = ctx->reps[rep];
+    if (!r.map_keys) SS_FAIL(ctx, SS_ERR_STATE, "map not built");
+    if (n == 0) return SS_OK;
+    for (int64_t i = 0; i < n; ++i) {
+        if (ids[i] == -1 || ids[i] == -2) SS_FAIL(ctx, SS_ERR_RANGE, "ids -1 and -2 are reserved");
+        if (slots[i] < 0 || slots[i] >= r.n_total) SS_FAIL(ctx, SS_ERR_RANGE, "slot %d outside the replica", slots[i]);
+    }
+    if (r.map_size + n > r.map_cap) SS_FAIL(ctx, SS_ERR_FULL, "map capacity %lld exceeded", (long long)r.map_cap);
+    int64_t *d_ids = nullptr;
+    int32_t *d_slots = nullptr;
+    SS_HIP(ctx, hipMalloc((void **)&d_ids, sizeof(int64_t) * n));
+    hipError_t e = hipMalloc((void **)&d_slots, sizeof(int32_t) * n);
+    if (e != hipSuccess) { dfree(d_ids); SS_HIP(ctx, e); }
+    int32_t zero = 0, overflow = 0;
+    e = hipMemcpy(d_ids, ids, sizeof(int64_t) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_slots, slots, sizeof(int32_t) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ctx->d_counters, &zero, sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = ss_launch_map_insert(r.map_keys, r.map_vals, r.map_cap, d_ids, d_slots, n, ctx->d_counters, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(&overflow, ctx->d_counters, sizeof(int32_t), hipMemcpyDeviceToHost);
+    dfree(d_ids);
+    dfree(d_slots);
+    SS_HIP(ctx, e);
+    if (overflow) SS_FAIL(ctx, SS_ERR_FULL, "%d inserts found no free slot", overflow);
+    r.map_size += n;
+    return SS_OK;
+}
+
+int ss_map_remove(ss_ctx *ctx, int32_t rep, const int64_t *ids, int64_t n, int64_t *n_removed) {
+    if (!valid_rep(ctx, rep) || (n && !ids)) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    ss_replica &r = ctx->reps[rep];
+    if (!r.map_keys) SS_FAIL(ctx, SS_ERR_STATE, "map not built");
+    int32_t removed = 0;
+    if (n > 0) {
+        int64_t *d_ids = nullptr;
+        SS_HIP(ctx, hipMalloc((void **)&d_ids, sizeof(int64_t) * n));
+        int32_t zero = 0;
+        hipError_t e = hipMemcpy(d_ids, ids, sizeof(int64_t) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(ctx->d_counters, &zero, sizeof(int32_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = ss_launch_map_remove(r.map_keys, r.map_vals, r.map_cap, d_ids, n, ctx->d_counters, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(&removed, ctx->d_counters, sizeof(int32_t), hipMemcpyDeviceToHost);
+        dfree(d_ids);
+        SS_HIP(ctx, e);
+    }
+    r.map_size -= removed;
+    if (n_removed) *n_removed = removed;
+    return SS_OK;
+}
+
+static int lookup_dev(ss_ctx *ctx, ss_replica &r, const int64_t *ids, int32_t *d_out, int64_t n, int64_t *n_missing) {
+    int64_t *d_ids = nullptr;
+    SS_HIP(ctx, hipMalloc((void **)&d_ids, sizeof(int64_t) * n));
+    int32_t zero = 0, miss = 0;
+    hipError_t e = hipMemcpy(d_ids, ids, sizeof(int64_t) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ctx->d_counters, &zero, sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = ss_launch_map_lookup(r.map_keys, r.map_vals, r.map_cap, d_ids, d_out, n, ctx->d_counters, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(&miss, ctx->d_counters, sizeof(int32_t), hipMemcpyDeviceToHost);
+    dfree(d_ids);
+    SS_HIP(ctx, e);
+    if (n_missing) *n_missing = miss;
+    return SS_OK;
+}
+
+int ss_map_lookup(ss_ctx *ctx, int32_t rep, const int64_t *ids, int32_t *out, int64_t n, int64_t *n_missing) {
+    if (!valid_rep(ctx, rep) || (n && (!ids || !out))) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    ss_replica &r = ctx->reps[rep];
+    if (!r.map_keys) SS_FAIL(ctx, SS_ERR_STATE, "map not built");
+    if (n == 0) { if (n_missing) *n_missing = 0; return SS_OK; }
+    int32_t *d_out = nullptr;
+    SS_HIP(ctx, hipMalloc((void **)&d_out, sizeof(int32_t) * n));
+    int rc = lookup_dev(ctx, r, ids, d_out, n, n_missing);
+    if (rc == SS_OK) {
+        hipError_t e = hipMemcpy(out, d_out, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { dfree(d_out); SS_HIP(ctx, e); }
+    }
+    dfree(d_out);
+    return rc;
+}
+
+int ss_halo_bind(ss_ctx *ctx, int32_t rep, const int64_t *ids, int64_t n, int64_t *n_missing) {
+    if (!valid_rep(ctx, rep) || (n && !ids)) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    ss_replica &r = ctx->reps[rep];
+    if (!r.map_keys) SS_FAIL(ctx, SS_ERR_STATE, "map not built");
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    dfree(r.halo_slots); dfree(r.halo_dev[0]); dfree(r.halo_dev[1]);
+    r.halo_slots = nullptr; r.halo_dev[0] = r.halo_dev[1] = nullptr;
+    r.n_halo = 0;
+    if (n == 0) return SS_OK;
+    SS_HIP(ctx, dalloc(ctx, &r, &r.halo_slots, (size_t)n));
+    SS_HIP(ctx, dalloc(ctx, &r, &r.halo_dev[0], (size_t)n));
+    SS_HIP(ctx, dalloc(ctx, &r, &r.halo_dev[1], (size_t)n));
+    int64_t miss = 0;
+    int rc = lookup_dev(ctx, r, ids, r.halo_slots, n, &miss);
+    if (rc) return rc;
+    if (n_missing) *n_missing = miss;
+    r.n_halo = n;
+    return SS_OK;
+}
+
+static int refresh_ghost_bits(ss_ctx *ctx, ss_replica &r) {
+    const int par = (int)(r.tick & 1);
+    return rebuild_bits(ctx, r, par, r.n_local / 64, r.n_words);
+}
+
+int ss_halo_unpack(ss_ctx *ctx, int32_t rep, const int8_t *vals, int64_t n) {
+    if (!valid_rep(ctx, rep) || (n && !vals)) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    ss_replica &r = ctx->reps[rep];
+    if (n != r.n_halo) SS_FAIL(ctx, SS_ERR_INVALID, "n = %lld, bound halo has %lld records", (long long)n, (long long)r.n_halo);
+    if (n == 0) return SS_OK;
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SS_HIP(ctx, hipMemcpy(r.halo_dev[0], vals, (size_t)n, hipMemcpyHostToDevice));
+    SS_HIP(ctx, ss_launch_halo_scatter(r.state[0], r.state[1], r.halo_slots, r.halo_dev[0], n, ctx->stream));
+    return refresh_ghost_bits(ctx, r);
+}
+
+int ss_halo_unpack_ids(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int8_t *vals, int64_t n, int64_t *n_missing) {
+    if (!valid_rep(ctx, rep) || (n && (!ids || !vals))) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    ss_replica &r = ctx->reps[rep];
+    if (!r.map_keys) SS_FAIL(ctx, SS_ERR_STATE, "map not built");
+    if (n == 0) { if (n_missing) *n_missing = 0; return SS_OK; }
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int64_t *d_ids = nullptr;
+    int8_t *d_vals = nullptr;
+    SS_HIP(ctx, hipMalloc((void **)&d_ids, sizeof(int64_t) * n));
+    hipError_t e = hipMalloc((void **)&d_vals, (size_t)n);
+    if (e != hipSuccess) { dfree(d_ids); SS_HIP(ctx, e); }
+    int32_t zero = 0, miss = 0;
+    e = hipMemcpy(d_ids, ids, sizeof(int64_t) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_vals, vals, (size_t)n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ctx->d_counters, &zero, sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = ss_launch_halo_unpack_ids(r.map_keys, r.map_vals, r.map_cap, d_ids, d_vals, n, r.state[0], r.state[1], ctx->d_counters, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(&miss, ctx->d_counters, sizeof(int32_t), hipMemcpyDeviceToHost);
+    dfree(d_ids);
+    dfree(d_vals);
+    SS_HIP(ctx, e);
+    if (n_missing) *n_missing = miss;
+    return refresh_ghost_bits(ctx, r);
+}
+
+int ss_halo_pack(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n, int8_t *out) {
+    if (!valid_rep(ctx, rep) || (n && (!slots || !out))) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    ss_replica &r = ctx->reps[rep];
+    for (int64_t i = 0; i < n; ++i)
+        if (slots[i] < 0 || slots[i] >= r.n_total) SS_FAIL(ctx, SS_ERR_RANGE, "slot %d outside the replica", slots[i]);
+    if (n == 0) return SS_OK;
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int32_t *d_slots = nullptr;
+    int8_t *d_out = nullptr;
+    SS_HIP(ctx, hipMalloc((void **)&d_slots, sizeof(int32_t) * n));
+    hipError_t e = hipMalloc((void **)&d_out, (size_t)n);
+    if (e != hipSuccess) { dfree(d_slots); SS_HIP(ctx, e); }
+    e = hipMemcpy(d_slots, slots, sizeof(int32_t) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = ss_launch_gather(r.state[r.tick & 1], d_slots, d_out, n, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d_out, (size_t)n, hipMemcpyDeviceToHost);
+    dfree(d_slots);
+    dfree(d_out);
+    SS_HIP(ctx, e);
+    return SS_OK;
+}
+
+int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals) {
+    if (!valid_rep(ctx, rep) || (ticks > 0 && !vals)) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    if (ticks < 0 || ticks > INT32_MAX) SS_FAIL(ctx, SS_ERR_INVALID, "ticks out of range");
+    ss_replica &r = ctx->reps[rep];
+    const int64_t n = r.n_halo;
+    if (n == 0) return ss_simulate(ctx, ticks);
+    int rc;
+    if (ctx->tiles_dirty && (rc = build_tiles(ctx))) return rc;
+    if (ctx->h_halo_cap < n) {
+        SS_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));
+        for (int b = 0; b < 2; ++b) {
+            if (ctx->h_halo_pin[b]) (void)hipHostFree(ctx->h_halo_pin[b]);
+            ctx->h_halo_pin[b] = nullptr;
+        }
+        for (int b = 0; b < 2; ++b)
+            SS_HIP(ctx, hipHostMalloc((void **)&ctx->h_halo_pin[b], (size_t)n, hipHostMallocDefault));
+        ctx->h_halo_cap = n;
+    }
+    if ((rc = push_descriptors(ctx, ticks))) return rc;
+    const int64_t w0 = r.n_local / 64;
+    for (int64_t k = 0; k < ticks; ++k) {
+        const int b = (int)(k & 1);
+        // the pinned slot is free once its previous copy finished
+        SS_HIP(ctx, hipEventSynchronize(ctx->halo_ev[b]));
+        std::memcpy(ctx->h_halo_pin[b], vals + (size_t)k * (size_t)n, (size_t)n);
+        SS_HIP(ctx, hipMemcpyAsync(r.halo_dev[b], ctx->h_halo_pin[b], (size_t)n, hipMemcpyHostToDevice, ctx->copy_stream));
+        SS_HIP(ctx, hipEventRecord(ctx->halo_ev[b], ctx->copy_stream));
+        SS_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->halo_ev[b], 0));
+        SS_HIP(ctx, ss_launch_halo_scatter(r.state[0], r.state[1], r.halo_slots, r.halo_dev[b], n, ctx->stream));
+        const int par = (int)((r.tick + k) & 1);
+        SS_HIP(ctx, ss_launch_bits(r.state[par], r.ibits[par], w0, r.n_words, r.n_total, r.dev.inf_code, ctx->stream));
+        SS_HIP(ctx, ss_launch_tick(ctx->d_reps, ctx->d_tiles, ctx->n_tiles, (int)k, ctx->apl, ctx->grid, ctx->stream));
+        // halo_dev[b] is rewritten two ticks later: the copy stream waits for this tick's scatter
+        hipEvent_t done;
+        SS_HIP(ctx, hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        SS_HIP(ctx, hipEventRecord(done, ctx->stream));
+        SS_HIP(ctx, hipStreamWaitEvent(ctx->copy_stream, done, 0));
+        (void)hipEventDestroy(done);
+    }
+    advance_ticks(ctx, ticks);
+    return SS_OK;
+}
+
+// ---------------- measurement ----------------
+
+int ss_set_counting(ss_ctx *ctx, int32_t on) {
+    if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->counting = on != 0;
+    return SS_OK;
+}
+
+int ss_read_counters(ss_ctx *ctx, int64_t *out) {
+    if (!ctx || !out) SS_FAIL(ctx, SS_ERR_INVALID, "NULL argument");
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    unsigned long long h[4];
+    SS_HIP(ctx, hipMemcpy(h, ctx->d_work, sizeof h, hipMemcpyDeviceToHost));
+    SS_HIP(ctx, hipMemset(ctx->d_work, 0, sizeof h));
+    for (int i = 0; i < 4; ++i) out[i] = (int64_t)h[i];
+    return SS_OK;
+}
+
+int ss_census(ss_ctx *ctx, int32_t rep, int64_t *out) {
+    if (!valid_rep(ctx, rep) || !out) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    ss_replica &r = ctx->reps[rep];
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SS_HIP(ctx, hipMemset(ctx->d_stats, 0, 8 * sizeof(unsigned long long)));
+    SS_HIP(ctx, ss_launch_stats(r.state[r.tick & 1], r.row_ptr, r.n_local, r.dev.s_code, ctx->d_stats,
+                                ctx->num_cus * 8, ctx->stream));
+    unsigned long long h[8];
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SS_HIP(ctx, hipMemcpy(h, ctx->d_stats, sizeof h, hipMemcpyDeviceToHost));
+    for (int i = 0; i < 5; ++i) out[i] = (int64_t)h[i];
+    return SS_OK;
+}
+
+int ss_timer_start(ss_ctx *ctx) {
+    if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
+    SS_HIP(ctx, hipEventRecord(ctx->t_start, ctx->stream));
+    return SS_OK;
+}
+
+int ss_timer_stop(ss_ctx *ctx, float *ms) {
+    if (!ctx || !ms) SS_FAIL(ctx, SS_ERR_INVALID, "NULL argument");
+    SS_HIP(ctx, hipEventRecord(ctx->t_stop, ctx->stream));
+    SS_HIP(ctx, hipEventSynchronize(ctx->t_stop));
+    SS_HIP(ctx, hipEventElapsedTime(ms, ctx->t_start, ctx->t_stop));
+    return SS_OK;
+}
+
+}  // extern "C"

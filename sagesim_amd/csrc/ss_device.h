@@ -1,0 +1,105 @@
+// Device-side data layout and inline device functions of the per-tick agent
+// step (gfx950).  Shared by the fused tick kernels, the halo kernels and the
+// hipRTC-compiled step functions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SS_HIST_W 32
+#define SS_U23_MASK 0x7FFFFFu
+
+// A replica is one independent simulation: its CSR, its int8 state ping-pong
+// buffers, its infectious-neighbour bitmaps and its compartment model.  The
+// plain path runs one replica; the ensemble and the serving loop run many in
+// the same launch.  Slots [0, n_local) are stepped; [n_local, n_total) are
+// ghost slots whose state is carried over and refreshed from the halo.
+struct ss_rep_dev {
+    const int64_t *row_ptr;     // [n_total + 1]; ghost rows are empty
+    const int32_t *col_idx;     // [nnz] slot indices
+    const uint32_t *rng_lo;     // [n_local] low word of the agent's RNG key id
+    const uint32_t *rng_hi;     // [n_local] high word, or null when all ids < 2^32
+    int8_t *state[2];           // ping-pong; tick t reads state[t & 1]
+    uint64_t *ibits[2];         // bit j of word w: state[.][64w + j] == inf_code
+    int32_t *history;           // [n_local][SS_HIST_W] or null
+    const uint32_t *inf_T;      // inf_T[k] = ceil(p_k * 2^23), k = #infectious neighbours
+    int32_t inf_T_last;         // last index of inf_T (k is clamped to it)
+    int32_t pad0;
+    uint32_t trans_T[4];        // spontaneous transition thresholds per state (0 = none)
+    int8_t trans_next[4];       // ...and target states
+    int8_t s_code, inf_code, s_target, n_states;
+    uint32_t seed_lo, seed_hi;
+    int64_t tick0;              // tick of launch step k = 0
+    int64_t tick_end;           // ticks >= tick_end are not run (replica finished)
+    int64_t n_local, n_total;
+    unsigned long long *counters;   // null, or [4] work counters (see tick_kernels.hip)
+    int64_t pad1;
+};
+
+// A tile is a 64-aligned agent range of one replica.  LIGHT tiles map one
+// agent to one lane; HEAVY tiles map one agent to one wavefront.
+enum { SS_TILE_LIGHT = 0, SS_TILE_HEAVY = 1 };
+struct ss_tile {
+    int32_t rep;
+    int32_t kind;
+    int64_t begin;
+    int64_t end;      // exclusive; end - begin <= tile capacity
+    int64_t pad;
+};
+
+// Philox-4x32-10 (Salmon et al., SC'11), word 0 only.
+__host__ __device__ __forceinline__ uint32_t ss_philox4x32_w0(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                              uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    return c0;
+}
+
+// The draw of agent `id` at tick t: key = seed, counter = (t, id) as 64-bit
+// halves; 23 low bits of word 0 (the reference's float32-mantissa conversion,
+// math_utils.py:189, kept as the integer numerator m of u = m / 2^23).
+__device__ __forceinline__ uint32_t ss_draw(const ss_rep_dev &R, int64_t t, int64_t i) {
+    uint32_t hi = R.rng_hi ? R.rng_hi[i] : 0u;
+    return ss_philox4x32_w0((uint32_t)t, (uint32_t)((uint64_t)t >> 32), R.rng_lo[i], hi, R.seed_lo,
+                            R.seed_hi) & SS_U23_MASK;
+}
+
+// Reference Philox-2x32-10 word 0 (math_utils.py:106-186): M = 0xD256D193,
+// key bump 0x9E3779B9.
+__host__ __device__ __forceinline__ uint32_t ss_philox2x32_ref(uint32_t key, uint32_t c0, uint32_t c1) {
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        uint64_t p = (uint64_t)c0 * 0xD256D193u;
+        uint32_t n0 = (uint32_t)(p >> 32) ^ key ^ c1;
+        c1 = (uint32_t)p;
+        key += 0x9E3779B9u;
+        c0 = n0;
+    }
+    uint64_t p = (uint64_t)c0 * 0xD256D193u;
+    return (uint32_t)(p >> 32) ^ key ^ c1;
+}
+
+__device__ __forceinline__ bool ss_bit(const uint64_t *bits, int64_t slot) {
+    return (bits[slot >> 6] >> (slot & 63)) & 1ull;
+}
+
+// Append `who` to a -1 padded history row (first free entry; when the row is
+// full the oldest entry is dropped).
+// Rare path (once per infection): kept as rolled loops so it does not raise
+// the register budget of the kernels that inline it.
+__device__ __forceinline__ void ss_hist_append(int32_t *row, int32_t who) {
+#pragma unroll 1
+    for (int j = 0; j < SS_HIST_W; ++j)
+        if (row[j] == -1) { row[j] = who; return; }
+#pragma unroll 1
+    for (int j = 0; j < SS_HIST_W - 1; ++j) row[j] = row[j + 1];
+    row[SS_HIST_W - 1] = who;
+}

@@ -1,0 +1,99 @@
+// Host-side runtime structures of libsagestep (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ss_device.h"
+
+#define SS_BLOCK 256
+
+// kernel launchers (tick_kernels.hip)
+hipError_t ss_launch_tick(const ss_rep_dev *reps, const ss_tile *tiles, int n_tiles, int k, int apl, int grid,
+                          hipStream_t st);
+hipError_t ss_occupancy_tick(int apl, int *blocks_per_cu);
+hipError_t ss_launch_bits(const int8_t *st, uint64_t *bits, int64_t w0, int64_t w1, int64_t n_total, int8_t inf,
+                          hipStream_t s);
+hipError_t ss_launch_map_clear(unsigned long long *keys, int32_t *vals, int64_t cap, hipStream_t s);
+hipError_t ss_launch_map_insert(unsigned long long *keys, int32_t *vals, int64_t cap, const int64_t *ids,
+                                const int32_t *slots, int64_t n, int32_t *overflow, hipStream_t s);
+hipError_t ss_launch_map_lookup(const unsigned long long *keys, const int32_t *vals, int64_t cap, const int64_t *ids,
+                                int32_t *out, int64_t n, int32_t *misses, hipStream_t s);
+hipError_t ss_launch_map_remove(unsigned long long *keys, int32_t *vals, int64_t cap, const int64_t *ids, int64_t n,
+                                int32_t *removed, hipStream_t s);
+hipError_t ss_launch_halo_scatter(int8_t *s0, int8_t *s1, const int32_t *slots, const int8_t *vals, int64_t n,
+                                  hipStream_t s);
+hipError_t ss_launch_halo_unpack_ids(const unsigned long long *keys, const int32_t *mvals, int64_t cap,
+                                     const int64_t *ids, const int8_t *vals, int64_t n, int8_t *s0, int8_t *s1,
+                                     int32_t *misses, hipStream_t s);
+hipError_t ss_launch_gather(const int8_t *st, const int32_t *slots, int8_t *out, int64_t n, hipStream_t s);
+hipError_t ss_launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s);
+hipError_t ss_launch_stats(const int8_t *st, const int64_t *row_ptr, int64_t n_local, int8_t s_code,
+                           unsigned long long *out, int grid, hipStream_t s);
+
+// One replica's device allocations and host bookkeeping.
+struct ss_replica {
+    bool live = false;
+    int64_t n_local = 0, n_ghost = 0, n_total = 0, nnz = 0;
+    int64_t n_words = 0;                       // bitmap words per buffer
+    // device memory (owned)
+    int64_t *row_ptr = nullptr;
+    int32_t *col_idx = nullptr;
+    uint32_t *rng_lo = nullptr, *rng_hi = nullptr;
+    int8_t *state[2] = {nullptr, nullptr};
+    int8_t *init_state = nullptr;
+    uint64_t *ibits[2] = {nullptr, nullptr};
+    int32_t *history = nullptr, *init_history = nullptr;
+    uint32_t *inf_T = nullptr;
+    // id -> slot map for ghosts
+    unsigned long long *map_keys = nullptr;
+    int32_t *map_vals = nullptr;
+    int64_t map_cap = 0, map_size = 0;
+    // bound halo: slot of every record of the per-tick halo buffer
+    int32_t *halo_slots = nullptr;
+    int64_t n_halo = 0;
+    int8_t *halo_dev[2] = {nullptr, nullptr};  // device staging of halo values
+    // bookkeeping
+    int64_t tick = 0, tick_init = 0;
+    int64_t tick_limit = INT64_MAX;            // replica finishes at this tick
+    int64_t max_deg = 0;
+    std::vector<int64_t> heavy_prefix_deg;      // thresholds
+    std::vector<int64_t> heavy_prefix_len;      // rows at the front with deg >= threshold
+    ss_rep_dev dev{};                           // device descriptor mirror (pointers filled)
+    size_t bytes = 0;                           // device bytes held
+};
+
+struct ss_ctx {
+    int device = 0;
+    int num_cus = 0;
+    int64_t total_mem = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;
+    std::vector<ss_replica> reps;
+    ss_rep_dev *d_reps = nullptr;              // device descriptor table
+    ss_rep_dev *h_reps = nullptr;              // pinned host mirror
+    int reps_cap = 0;
+    ss_tile *d_tiles = nullptr;
+    int64_t tiles_cap = 0;
+    int n_tiles = 0;
+    bool tiles_dirty = true;
+    // tiling (autotuned by the host library)
+    int heavy_deg = 64;
+    int apl = 1;
+    int grid_blocks = 0;                        // 0: persistent grid from occupancy
+    int use_graph = 1;
+    int grid = 0;
+    std::map<int64_t, hipGraphExec_t> graphs;   // by tick count
+    int32_t *d_counters = nullptr;              // misc device counters (misses, overflow)
+    unsigned long long *d_work = nullptr;       // [4] tick-kernel work counters
+    unsigned long long *d_stats = nullptr;      // [8] census output
+    bool counting = false;
+    hipEvent_t t_start = nullptr, t_stop = nullptr;
+    int8_t *h_halo_pin[2] = {nullptr, nullptr}; // pinned staging for per-tick halo values
+    int64_t h_halo_cap = 0;
+    hipEvent_t halo_ev[2] = {nullptr, nullptr};
+    std::string err;
+    size_t bytes = 0;
+};

@@ -1,0 +1,298 @@
+"""Python face of libsagestep: a context of device-resident replicas.
+
+`Engine` is a thin owner of one `ss_ctx`.  `Replica` adds the degree-sorted
+relabelling: the engine stores rows in descending-degree order (so one
+wavefront takes each high-degree agent and lanes take the rest), while the
+caller keeps its own agent order; the RNG stays keyed on the caller's ids, so
+results do not depend on the relabelling.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import (HISTORY_WIDTH, NativeError, SSModel, SSReplicaSpec, ptr, step_lib)
+from .graphs import CSRGraph, degree_sort
+
+S, I, R = 0, 1, 2
+
+
+def sir_model(beta, gamma):
+    """SIR {S=0, I=1, R=2}: S -> I with prob 1-(1-beta)^k, I -> R with prob gamma."""
+    m = SSModel()
+    m.n_states, m.s_code, m.inf_code, m.s_target = 3, 0, 1, 1
+    m.beta = float(beta)
+    m.trans_p[:] = [0.0, float(gamma), 0.0, 0.0]
+    m.trans_next[:] = [0, 2, 2, 2]
+    return m
+
+
+def seir_model(beta, sigma, gamma):
+    """SEIR {S=0, E=1, I=2, R=3}: S -> E (1-(1-beta)^k, k = #I), E -> I (sigma), I -> R (gamma)."""
+    m = SSModel()
+    m.n_states, m.s_code, m.inf_code, m.s_target = 4, 0, 2, 1
+    m.beta = float(beta)
+    m.trans_p[:] = [0.0, float(sigma), float(gamma), 0.0]
+    m.trans_next[:] = [0, 2, 3, 3]
+    return m
+
+
+class Engine:
+    """One libsagestep context on one GPU."""
+
+    def __init__(self, device=0):
+        self._L = step_lib()
+        h = ctypes.c_void_p()
+        rc = self._L.ss_create(int(device), ctypes.byref(h))
+        if rc != 0:
+            raise NativeError(rc, self._L.ss_last_error(None).decode())
+        self._h = h
+        self.device = device
+
+    # -- plumbing --
+    def _ck(self, rc):
+        if rc != 0:
+            raise NativeError(rc, self._L.ss_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ss_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def info(self):
+        cus, mem, used = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64()
+        self._ck(self._L.ss_device_info(self._h, ctypes.byref(cus), ctypes.byref(mem), ctypes.byref(used)))
+        return {"num_cus": cus.value, "total_mem": mem.value, "ctx_bytes": used.value}
+
+    # -- replicas --
+    def add_replica(self, row_ptr, col_idx, rng_key_id, state, model, seed, n_ghost=0,
+                    history=None, tick0=0, tick_limit=0):
+        rp = np.ascontiguousarray(row_ptr, np.int64)
+        ci = np.ascontiguousarray(col_idx, np.int32)
+        ids = np.ascontiguousarray(rng_key_id, np.int64)
+        st = np.ascontiguousarray(state, np.int8)
+        n_local = rp.shape[0] - 1
+        if ids.shape[0] != n_local or st.shape[0] != n_local + n_ghost:
+            raise ValueError("rng_key_id must have n_local entries and state n_local + n_ghost")
+        hist = None
+        if history is not None:
+            hist = np.ascontiguousarray(history, np.int32)
+            if hist.shape != (n_local, HISTORY_WIDTH):
+                raise ValueError(f"history must be ({n_local}, {HISTORY_WIDTH}) int32")
+        sp = SSReplicaSpec()
+        sp.n_local, sp.n_ghost, sp.nnz = n_local, int(n_ghost), ci.shape[0]
+        sp.row_ptr, sp.col_idx, sp.rng_key_id, sp.state = ptr(rp), ptr(ci), ptr(ids), ptr(st)
+        sp.history = ptr(hist) if hist is not None else None
+        sp.model = model
+        sp.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        sp.tick0 = int(tick0)
+        sp.tick_limit = int(tick_limit)
+        rep = ctypes.c_int32()
+        self._ck(self._L.ss_replica_create(self._h, ctypes.byref(sp), ctypes.byref(rep)))
+        return rep.value
+
+    def remove_replica(self, rep):
+        self._ck(self._L.ss_replica_destroy(self._h, rep))
+
+    def reset(self, rep):
+        self._ck(self._L.ss_replica_reset(self._h, rep))
+
+    def tick(self, rep):
+        t = ctypes.c_int64()
+        self._ck(self._L.ss_replica_tick(self._h, rep, ctypes.byref(t)))
+        return t.value
+
+    def set_tick(self, rep, tick, tick_limit=0):
+        self._ck(self._L.ss_replica_set_tick(self._h, rep, int(tick), int(tick_limit)))
+
+    def replica_bytes(self, rep):
+        b = ctypes.c_int64()
+        self._ck(self._L.ss_replica_bytes(self._h, rep, ctypes.byref(b)))
+        return b.value
+
+    def set_model(self, rep, model):
+        self._ck(self._L.ss_set_model(self._h, rep, ctypes.byref(model)))
+
+    def set_seed(self, rep, seed):
+        self._ck(self._L.ss_set_seed(self._h, rep, int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    def state(self, rep, n_total):
+        out = np.empty(n_total, np.int8)
+        self._ck(self._L.ss_state_download(self._h, rep, ptr(out), n_total))
+        return out
+
+    def set_state(self, rep, st):
+        st = np.ascontiguousarray(st, np.int8)
+        self._ck(self._L.ss_state_upload(self._h, rep, ptr(st), st.shape[0]))
+
+    def history(self, rep, n_local):
+        out = np.empty((n_local, HISTORY_WIDTH), np.int32)
+        self._ck(self._L.ss_history_download(self._h, rep, ptr(out), n_local))
+        return out
+
+    # -- run --
+    def set_tiling(self, heavy_degree=64, agents_per_lane=1, grid_blocks=0, use_graph=True):
+        self._ck(self._L.ss_set_tiling(self._h, int(heavy_degree), int(agents_per_lane), int(grid_blocks),
+                                       1 if use_graph else 0))
+
+    def launch_shape(self):
+        nt, g = ctypes.c_int32(), ctypes.c_int32()
+        self._ck(self._L.ss_launch_shape(self._h, ctypes.byref(nt), ctypes.byref(g)))
+        return nt.value, g.value
+
+    def simulate(self, ticks):
+        self._ck(self._L.ss_simulate(self._h, int(ticks)))
+
+    def sync(self):
+        self._ck(self._L.ss_sync(self._h))
+
+    # -- measurement --
+    def set_counting(self, on):
+        self._ck(self._L.ss_set_counting(self._h, 1 if on else 0))
+
+    def read_counters(self):
+        """(susceptible agents visited, draws, scans started, CSR entries read); clears them."""
+        out = np.zeros(4, np.int64)
+        self._ck(self._L.ss_read_counters(self._h, ptr(out)))
+        return out
+
+    def census(self, rep):
+        """(#S, #I or E, #R or I, #state 3, sum of degrees of susceptible agents)."""
+        out = np.zeros(5, np.int64)
+        self._ck(self._L.ss_census(self._h, rep, ptr(out)))
+        return out
+
+    def timer_start(self):
+        self._ck(self._L.ss_timer_start(self._h))
+
+    def timer_stop(self):
+        """Milliseconds between timer_start and now on the context's stream (waits)."""
+        ms = ctypes.c_float()
+        self._ck(self._L.ss_timer_stop(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    # -- ghosts --
+    def map_build(self, rep, ids, slots, load_factor=0.5):
+        ids = np.ascontiguousarray(ids, np.int64)
+        slots = np.ascontiguousarray(slots, np.int32)
+        self._ck(self._L.ss_map_build(self._h, rep, ptr(ids), ptr(slots), ids.shape[0], float(load_factor)))
+
+    def map_insert(self, rep, ids, slots):
+        ids = np.ascontiguousarray(ids, np.int64)
+        slots = np.ascontiguousarray(slots, np.int32)
+        self._ck(self._L.ss_map_insert(self._h, rep, ptr(ids), ptr(slots), ids.shape[0]))
+
+    def map_remove(self, rep, ids):
+        ids = np.ascontiguousarray(ids, np.int64)
+        n = ctypes.c_int64()
+        self._ck(self._L.ss_map_remove(self._h, rep, ptr(ids), ids.shape[0], ctypes.byref(n)))
+        return n.value
+
+    def map_lookup(self, rep, ids):
+        ids = np.ascontiguousarray(ids, np.int64)
+        out = np.empty(ids.shape[0], np.int32)
+        miss = ctypes.c_int64()
+        self._ck(self._L.ss_map_lookup(self._h, rep, ptr(ids), ptr(out), ids.shape[0], ctypes.byref(miss)))
+        return out, miss.value
+
+    def halo_bind(self, rep, ids):
+        ids = np.ascontiguousarray(ids, np.int64)
+        miss = ctypes.c_int64()
+        self._ck(self._L.ss_halo_bind(self._h, rep, ptr(ids), ids.shape[0], ctypes.byref(miss)))
+        return miss.value
+
+    def halo_unpack(self, rep, vals):
+        vals = np.ascontiguousarray(vals, np.int8)
+        self._ck(self._L.ss_halo_unpack(self._h, rep, ptr(vals), vals.shape[0]))
+
+    def halo_unpack_ids(self, rep, ids, vals):
+        ids = np.ascontiguousarray(ids, np.int64)
+        vals = np.ascontiguousarray(vals, np.int8)
+        miss = ctypes.c_int64()
+        self._ck(self._L.ss_halo_unpack_ids(self._h, rep, ptr(ids), ptr(vals), ids.shape[0], ctypes.byref(miss)))
+        return miss.value
+
+    def halo_pack(self, rep, slots):
+        slots = np.ascontiguousarray(slots, np.int32)
+        out = np.empty(slots.shape[0], np.int8)
+        self._ck(self._L.ss_halo_pack(self._h, rep, ptr(slots), slots.shape[0], ptr(out)))
+        return out
+
+    def simulate_halo(self, rep, ticks, vals):
+        vals = np.ascontiguousarray(vals, np.int8)
+        self._ck(self._L.ss_simulate_halo(self._h, rep, int(ticks), ptr(vals)))
+
+
+class Replica:
+    """A graph + compartment model placed on an Engine in degree-sorted order.
+
+    States and history are exchanged in the caller's agent order.  Slots at
+    and beyond graph.n are ghosts (graph.n_cols - graph.n of them).
+    """
+
+    def __init__(self, engine, graph: CSRGraph, state, model, seed, rng_key_id=None,
+                 history=None, tick0=0, tick_limit=0, sort=True):
+        self.engine = engine
+        self.n = graph.n
+        self.n_ghost = graph.n_cols - graph.n
+        self.n_total = graph.n_cols
+        state = np.asarray(state, np.int8)
+        if state.shape[0] != self.n_total:
+            raise ValueError("state must cover local and ghost slots")
+        if rng_key_id is None:
+            rng_key_id = np.arange(self.n, dtype=np.int64)
+        rng_key_id = np.asarray(rng_key_id, np.int64)
+        if sort:
+            gs, perm, inv = degree_sort(graph)
+        else:
+            gs, perm, inv = graph, np.arange(self.n, dtype=np.int32), np.arange(self.n, dtype=np.int32)
+        self.graph, self.perm, self.inv = gs, perm, inv
+        st = np.concatenate([state[:self.n][perm], state[self.n:]])
+        hist = None
+        if history is not None:
+            hist = self._map_hist(np.asarray(history, np.int32)[perm], inv)
+        self.rep = engine.add_replica(gs.row_ptr, gs.col_idx, rng_key_id[perm], st, model, seed,
+                                      n_ghost=self.n_ghost, history=hist, tick0=tick0, tick_limit=tick_limit)
+
+    def _map_hist(self, h, table):
+        h = h.copy()
+        local = (h >= 0) & (h < self.n)
+        h[local] = table[h[local]]
+        return np.ascontiguousarray(h, np.int32)
+
+    def states(self):
+        st = self.engine.state(self.rep, self.n_total)
+        out = np.empty_like(st)
+        out[self.perm] = st[:self.n]
+        out[self.n:] = st[self.n:]
+        return out
+
+    def set_states(self, state):
+        state = np.asarray(state, np.int8)
+        self.engine.set_state(self.rep, np.concatenate([state[:self.n][self.perm], state[self.n:]]))
+
+    def history(self):
+        h = self.engine.history(self.rep, self.n)
+        out = np.empty_like(h)
+        out[self.perm] = h
+        return self._map_hist(out, self.perm)
+
+    def tick(self):
+        return self.engine.tick(self.rep)
+
+    def reset(self):
+        self.engine.reset(self.rep)
+
+    def remove(self):
+        self.engine.remove_replica(self.rep)

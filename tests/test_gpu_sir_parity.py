@@ -1,0 +1,188 @@
+"""Bit-exact parity of the fused HIP tick kernels against the CPU oracle.
+
+Every test compares per-agent int8 states (and, where present, the
+last-infector history) produced through the C ABI with oracle/sir_oracle.c on
+the same seeded inputs.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sagesim_amd import Engine, Replica, graphs, sir_model, seir_model, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def oracle_model(w):
+    if w.sigma > 0:
+        return O.seir_model(w.beta, w.sigma, w.gamma, w.seed)
+    return O.sir_model(w.beta, w.gamma, w.seed)
+
+
+def first_diff(a, b):
+    d = np.nonzero(a != b)[0]
+    return None if d.size == 0 else (int(d[0]), int(a[d[0]]), int(b[d[0]]), int(d.size))
+
+
+def test_er_sir_every_tick(eng):
+    """10k-agent G(n, 0.001) SIR, 100 ticks: identical states after every tick."""
+    w = workloads.er_sir()
+    say("er_sir every tick", w.graph.n, w.graph.nnz)
+    rep = Replica(eng, w.graph, w.state, w.model, w.seed)
+    _, trace = O.sir_run(oracle_model(w), w.graph.row_ptr, w.graph.col_idx, np.arange(w.graph.n),
+                         w.state, w.ticks, trace=True)
+    for t in range(w.ticks):
+        eng.simulate(1)
+        got = rep.states()
+        assert first_diff(got, trace[t]) is None, (t, first_diff(got, trace[t]))
+    # the epidemic did something
+    assert (trace[-1] == 2).sum() > 100
+    rep.remove()
+
+
+@pytest.mark.parametrize("heavy,apl,graph_loop", [(64, 1, True), (8, 2, True), (1 << 30, 4, False), (16, 1, False)])
+def test_ba_launch_shapes(eng, heavy, apl, graph_loop):
+    """BA(50k, 5), 60 ticks, in one captured loop: every launch shape gives the oracle's states."""
+    w = workloads.ba_sir(n=50_000, m=5, beta=0.06, ticks=60, seed=3)
+    say("ba launch shape", heavy, apl, graph_loop)
+    eng.set_tiling(heavy, apl, 0, graph_loop)
+    rep = Replica(eng, w.graph, w.state, w.model, w.seed)
+    want, _ = O.sir_run(oracle_model(w), w.graph.row_ptr, w.graph.col_idx, np.arange(w.graph.n),
+                        w.state, w.ticks)
+    eng.simulate(w.ticks)
+    got = rep.states()
+    assert first_diff(got, want) is None, first_diff(got, want)
+    assert rep.tick() == w.ticks
+    rep.remove()
+    eng.set_tiling(64, 1, 0, True)
+
+
+def test_ba_1m_subset_200_ticks(eng):
+    """BA(1M, 5), beta 0.03, gamma 0.1, 0.1% infected, 200 ticks: states compared every 25 ticks."""
+    w = workloads.ba_sir(n=1_000_000, m=5, ticks=200)
+    say("ba 1M subset", w.graph.nnz)
+    rep = Replica(eng, w.graph, w.state, w.model, w.seed)
+    st = w.state
+    om = oracle_model(w)
+    for t0 in range(0, 200, 25):
+        st, _ = O.sir_run(om, w.graph.row_ptr, w.graph.col_idx, np.arange(w.graph.n), st, 25, tick0=t0)
+        eng.simulate(25)
+        got = rep.states()
+        assert first_diff(got, st) is None, (t0 + 25, first_diff(got, st))
+    counts = np.bincount(st, minlength=3)
+    say("final S/I/R", counts.tolist())
+    assert counts[2] > 1000
+    rep.remove()
+
+
+def test_seir_ws(eng):
+    w = workloads.ws_seir(n=30_000, ticks=80)
+    say("seir ws", w.graph.n)
+    rep = Replica(eng, w.graph, w.state, w.model, w.seed)
+    want, _ = O.sir_run(oracle_model(w), w.graph.row_ptr, w.graph.col_idx, np.arange(w.graph.n),
+                        w.state, w.ticks)
+    eng.simulate(w.ticks)
+    got = rep.states()
+    assert first_diff(got, want) is None, first_diff(got, want)
+    assert (want == 3).sum() > 0
+    rep.remove()
+
+
+def test_edge_cases(eng):
+    """Isolated agents, a single agent, n not a multiple of 64, a hub of degree > 64*3,
+    beta = 1 (BFS front), gamma = 1, everything infected."""
+    say("edge cases")
+    rng = np.random.default_rng(5)
+    cases = []
+    # star hub with 300 leaves + 37 isolated agents
+    n = 338
+    u = np.zeros(300, np.int64)
+    v = np.arange(1, 301, dtype=np.int64)
+    cases.append(("hub", graphs.from_edges(n, u, v), 0.3, 0.2))
+    cases.append(("single", graphs.from_edges(1, [], []), 0.5, 0.5))
+    cases.append(("chain_beta1", graphs.from_edges(101, np.arange(100), np.arange(1, 101)), 1.0, 1.0))
+    g = graphs.erdos_renyi(1000, 0.01, 9)
+    cases.append(("er_gamma1", g, 0.2, 1.0))
+    for name, g, beta, gamma in cases:
+        for init in ("one", "all"):
+            st = np.zeros(g.n, np.int8)
+            if init == "one":
+                st[0] = 1
+            else:
+                st[:] = 1
+            say(name, init)
+            seed = int(rng.integers(1 << 62))
+            rep = Replica(eng, g, st, sir_model(beta, gamma), seed)
+            want, trace = O.sir_run(O.sir_model(beta, gamma, seed), g.row_ptr, g.col_idx, np.arange(g.n), st, 30,
+                                    trace=True)
+            for t in range(30):
+                eng.simulate(1)
+                got = rep.states()
+                assert first_diff(got, trace[t]) is None, (name, init, t, first_diff(got, trace[t]))
+            rep.remove()
+
+
+def test_history_and_ghosts(eng):
+    """One SBM partition with ghosts refreshed from a halo each tick and a
+    last-infector history: states and history identical to the oracle."""
+    pw = workloads.sbm_partition(n_blocks=16, block_size=4000, deg_in=12, deg_out=2, n_parts=4, part=1,
+                                 ticks=40, seed=17)
+    g = pw.graph
+    say("sbm partition", g.n, g.n_cols, g.nnz)
+    rep = Replica(eng, g, pw.state, sir_model(pw.beta, pw.gamma), pw.seed, rng_key_id=pw.local_ids,
+                  history=pw.history)
+    n_ghost = g.n_cols - g.n
+    eng.map_build(rep.rep, pw.ghost_ids, np.arange(g.n, g.n_cols, dtype=np.int32), 0.5)
+    assert eng.halo_bind(rep.rep, pw.halo_ids) == 0
+    vals = pw.halo_vals(0, pw.ticks)
+    hist = pw.history.copy()
+    want, _ = O.sir_run(O.sir_model(pw.beta, pw.gamma, pw.seed), g.row_ptr, g.col_idx, pw.local_ids,
+                        pw.state, pw.ticks, n_local=g.n, history=hist, ghost_ids=pw.ghost_ids,
+                        halo_ids=pw.halo_ids, halo_vals=vals)
+    eng.simulate_halo(rep.rep, pw.ticks, vals)
+    got = rep.states()
+    assert first_diff(got[:g.n], want[:g.n]) is None, first_diff(got[:g.n], want[:g.n])
+    gh = rep.history()
+    assert np.array_equal(gh, hist), np.argwhere(gh != hist)[:5]
+    assert (hist != pw.history).any()
+    assert n_ghost > 0
+    rep.remove()
+
+
+def test_hash_map_against_dict(eng):
+    """The GPU id -> slot map answers like a dict, including misses, removal and reinsertion."""
+    say("hash map")
+    g = graphs.from_edges(10, [], [])
+    rep = eng.add_replica(g.row_ptr, g.col_idx, np.arange(10), np.zeros(5000 + 10, np.int8), sir_model(0.1, 0.1), 1,
+                          n_ghost=5000)
+    rng = np.random.default_rng(3)
+    ids = rng.integers(0, 1 << 62, size=5000, dtype=np.int64)
+    ids = np.unique(ids)
+    slots = rng.integers(0, 5010, size=ids.shape[0]).astype(np.int32)
+    eng.map_build(rep, ids, slots, 0.5)
+    probe = np.concatenate([ids, rng.integers(0, 1 << 62, size=1000, dtype=np.int64)])
+    got, miss = eng.map_lookup(rep, probe)
+    d = dict(zip(ids.tolist(), slots.tolist()))
+    want = np.array([d.get(int(x), -1) for x in probe], np.int32)
+    assert np.array_equal(got, want)
+    assert miss == int((want < 0).sum())
+    rm = ids[::3]
+    assert eng.map_remove(rep, rm) == rm.shape[0]
+    got, _ = eng.map_lookup(rep, ids)
+    want = np.where(np.isin(ids, rm), -1, slots)
+    assert np.array_equal(got, want)
+    eng.map_insert(rep, rm, slots[::3])
+    got, _ = eng.map_lookup(rep, ids)
+    assert np.array_equal(got, slots)
+    eng.remove_replica(rep)
