@@ -70,8 +70,9 @@ def main():
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--m", type=int, default=5)
     ap.add_argument("--ticks", type=int, default=200)
+    ap.add_argument("--hub", type=int, default=1024, help="degree from which a row gets a whole block")
     ap.add_argument("--heavy", type=int, default=64, help="degree from which a row gets a whole wavefront")
-    ap.add_argument("--apl", type=int, default=1, help="agents per lane of the other rows")
+    ap.add_argument("--apl", type=int, default=4, help="agents per lane of the other rows")
     ap.add_argument("--grid", type=int, default=0, help="blocks of the persistent grid (0: from occupancy)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -93,7 +94,7 @@ def main():
     log(f"graph BA({g.n}, {args.m}): nnz {g.nnz}, max degree {int(g.degrees().max())}, {t_gen:.1f}s")
 
     eng = Engine(0)
-    eng.set_tiling(args.heavy, args.apl, args.grid, True)
+    eng.set_tiling(args.heavy, args.apl, args.grid, True, hub_degree=args.hub)
     t0 = time.perf_counter()
     rep = Replica(eng, g, w.state, w.model, w.seed)
     eng.sync()
@@ -199,7 +200,7 @@ def main():
         "config": {"workload": f"plain SIR, BA(n={g.n}, m={args.m}), beta={w.beta}, gamma={w.gamma}, "
                                f"{args.ticks} ticks, degree-sorted CSR (int64 row_ptr, int32 col_idx)",
                    "agents": g.n, "ticks": args.ticks, "nnz": g.nnz,
-                   "tiling": {"heavy_degree": args.heavy, "agents_per_lane": args.apl, "grid": grid,
+                   "tiling": {"hub_degree": args.hub, "heavy_degree": args.heavy, "agents_per_lane": args.apl, "grid": grid,
                               "tiles": n_tiles},
                    "graph_gen_s": round(t_gen, 2), "upload_s": round(t_upload, 2),
                    "final_SIR": counts},

@@ -116,12 +116,15 @@ int ss_history_download(ss_ctx *ctx, int32_t rep, int32_t *out, int64_t n_rows);
  * Advances every live replica by `ticks` ticks (each stops at its own
  * tick_limit).  Tick loops are captured once per tick count in a hipGraph. */
 int ss_simulate(ss_ctx *ctx, int64_t ticks);
-/* Launch shape: heavy_degree = rows with at least this degree at the front
- * of a replica get one wavefront each; agents_per_lane in {1,2,4} for the
- * other rows; grid_blocks = 0 for a persistent grid sized from occupancy;
- * use_graph = 0 launches tick by tick. */
-int ss_set_tiling(ss_ctx *ctx, int32_t heavy_degree, int32_t agents_per_lane, int32_t grid_blocks,
-                  int32_t use_graph);
+/* Launch shape.  Rows are expected in descending-degree order.  The rows of
+ * degree >= hub_degree at the front get a whole block each; the following
+ * rows of degree >= heavy_degree get a wavefront each for their neighbour
+ * scan; the rest run agents_per_lane (1, 2 or 4) agents per lane.  Degrees
+ * are one of 8,16,24,32,48,64,96,128,192,256,512,1024,2048,4096,16384,2^30.
+ * grid_blocks = 0 sizes the persistent grid from occupancy; use_graph = 0
+ * launches tick by tick instead of replaying a captured hipGraph. */
+int ss_set_tiling(ss_ctx *ctx, int32_t hub_degree, int32_t heavy_degree, int32_t agents_per_lane,
+                  int32_t grid_blocks, int32_t use_graph);
 /* tiles and grid of the current launch shape (after ss_simulate built them) */
 int ss_launch_shape(ss_ctx *ctx, int32_t *n_tiles, int32_t *grid);
 

@@ -83,7 +83,7 @@ def step_lib():
     _proto(L, "ss_state_upload", I, _P, _c_i32, _P, _c_i64)
     _proto(L, "ss_history_download", I, _P, _c_i32, _P, _c_i64)
     _proto(L, "ss_simulate", I, _P, _c_i64)
-    _proto(L, "ss_set_tiling", I, _P, _c_i32, _c_i32, _c_i32, _c_i32)
+    _proto(L, "ss_set_tiling", I, _P, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32)
     _proto(L, "ss_launch_shape", I, _P, _P, _P)
     _proto(L, "ss_map_build", I, _P, _c_i32, _P, _P, _c_i64, _c_dbl)
     _proto(L, "ss_map_insert", I, _P, _c_i32, _P, _P, _c_i64)

@@ -138,22 +138,33 @@ int64_t heavy_prefix(const ss_replica &r, int32_t thr) {
     return -1;
 }
 
+// Tiles of every live replica: one HUB tile per row of degree >= hub_deg at the
+// front, HEAVY tiles of SS_HEAVY_TILE agents up to the first 64-aligned slot
+// past the rows of degree >= heavy_deg, LIGHT tiles of 256 * apl agents for
+// the rest (light tiles start 64-aligned: they write whole bitmap words, the
+// hub and heavy tiles set single bits).  Blocks stride over the list, hubs
+// first, so each block starts with its share of the expensive tiles.
 int build_tiles(ss_ctx *ctx) {
-    std::vector<ss_tile> heavy, light;
+    std::vector<ss_tile> hubs, heavy, light;
     const int64_t cap = (int64_t)SS_BLOCK * ctx->apl;
     for (int32_t ri = 0; ri < (int32_t)ctx->reps.size(); ++ri) {
         ss_replica &r = ctx->reps[ri];
         if (!r.live || r.n_total == 0) continue;
-        int64_t H = heavy_prefix(r, ctx->heavy_deg);
-        if (H < 0) SS_FAIL(ctx, SS_ERR_INVALID, "heavy_degree %d has no prefix table", ctx->heavy_deg);
-        int64_t H64 = std::min<int64_t>((H + 63) / 64 * 64, r.n_total);
-        for (int64_t b = 0; b < H64; b += 64)
-            heavy.push_back({ri, SS_TILE_HEAVY, b, std::min<int64_t>(b + 64, r.n_total), 0});
+        const int64_t H = heavy_prefix(r, ctx->heavy_deg), Hh = heavy_prefix(r, ctx->hub_deg);
+        if (H < 0 || Hh < 0) SS_FAIL(ctx, SS_ERR_INVALID, "degree threshold without a prefix table");
+        const int64_t H64 = std::min<int64_t>((H + 63) / 64 * 64, r.n_total);
+        // hub count rounded up to 8: the 8 agents of a heavy wavefront then
+        // always share one bitmap word
+        const int64_t nh = std::min((Hh + 7) / 8 * 8, H64);
+        for (int64_t b = 0; b < nh; ++b) hubs.push_back({ri, SS_TILE_HUB, b, b + 1, 0});
+        for (int64_t b = nh; b < H64; b += SS_HEAVY_TILE)
+            heavy.push_back({ri, SS_TILE_HEAVY, b, std::min<int64_t>(b + SS_HEAVY_TILE, H64), 0});
         for (int64_t b = H64; b < r.n_total; b += cap)
             light.push_back({ri, SS_TILE_LIGHT, b, std::min<int64_t>(b + cap, r.n_total), 0});
     }
     std::vector<ss_tile> all;
-    all.reserve(heavy.size() + light.size());
+    all.reserve(hubs.size() + heavy.size() + light.size());
+    all.insert(all.end(), hubs.begin(), hubs.end());
     all.insert(all.end(), heavy.begin(), heavy.end());
     all.insert(all.end(), light.begin(), light.end());
     if ((int64_t)all.size() > INT32_MAX) SS_FAIL(ctx, SS_ERR_RANGE, "too many tiles");
@@ -205,7 +216,6 @@ int push_descriptors(ss_ctx *ctx, int64_t ticks) {
     for (size_t i = 0; i < ctx->reps.size(); ++i) {
         ss_replica &r = ctx->reps[i];
         ss_rep_dev d = r.dev;
-        d.counters = ctx->counting ? ctx->d_work : nullptr;
         if (r.live) {
             d.tick0 = r.tick;
             int64_t end = ticks > INT64_MAX - r.tick ? INT64_MAX : r.tick + ticks;
@@ -232,12 +242,16 @@ void advance_ticks(ss_ctx *ctx, int64_t ticks) {
         }
 }
 
+// launch step k of a tick loop
+hipError_t tick_launch(ss_ctx *ctx, int64_t k) {
+    return ss_launch_tick(ctx->d_reps, ctx->d_tiles, ctx->n_tiles, (int)k, ctx->apl, ctx->grid,
+                          ctx->counting ? ctx->d_work : nullptr, ctx->stream);
+}
+
 int launch_loop(ss_ctx *ctx, int64_t ticks) {
     if (ticks > INT32_MAX) SS_FAIL(ctx, SS_ERR_RANGE, "ticks per call must fit int32");
     if (!ctx->use_graph) {
-        for (int64_t k = 0; k < ticks; ++k)
-            SS_HIP(ctx, ss_launch_tick(ctx->d_reps, ctx->d_tiles, ctx->n_tiles, (int)k, ctx->apl, ctx->grid,
-                                       ctx->stream));
+        for (int64_t k = 0; k < ticks; ++k) SS_HIP(ctx, tick_launch(ctx, k));
         return SS_OK;
     }
     auto it = ctx->graphs.find(ticks);
@@ -245,8 +259,7 @@ int launch_loop(ss_ctx *ctx, int64_t ticks) {
         hipGraph_t g = nullptr;
         SS_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         hipError_t le = hipSuccess;
-        for (int64_t k = 0; k < ticks && le == hipSuccess; ++k)
-            le = ss_launch_tick(ctx->d_reps, ctx->d_tiles, ctx->n_tiles, (int)k, ctx->apl, ctx->grid, ctx->stream);
+        for (int64_t k = 0; k < ticks && le == hipSuccess; ++k) le = tick_launch(ctx, k);
         hipError_t ee = hipStreamEndCapture(ctx->stream, &g);
         SS_HIP(ctx, le);
         SS_HIP(ctx, ee);
@@ -376,7 +389,7 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
     r.n_words = (nt + 63) / 64;
     r.max_deg = max_deg;
     // heavy prefix lengths for the tiling thresholds the autotuner may pick
-    for (int32_t thr : {8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 512, 1024, 1 << 30}) {
+    for (int32_t thr : SS_THRESHOLDS) {
         int64_t H = 0;
         while (H < nl && sp->row_ptr[H + 1] - sp->row_ptr[H] >= thr) ++H;
         r.heavy_prefix_deg.push_back(thr);
@@ -567,15 +580,22 @@ int ss_history_download(ss_ctx *ctx, int32_t rep, int32_t *out, int64_t n_rows) 
     return SS_OK;
 }
 
-int ss_set_tiling(ss_ctx *ctx, int32_t heavy_degree, int32_t apl, int32_t grid_blocks, int32_t use_graph) {
+int ss_set_tiling(ss_ctx *ctx, int32_t hub_degree, int32_t heavy_degree, int32_t apl, int32_t grid_blocks,
+                  int32_t use_graph) {
     if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
     if (apl != 1 && apl != 2 && apl != 4) SS_FAIL(ctx, SS_ERR_INVALID, "agents_per_lane must be 1, 2 or 4");
-    static const int32_t ok[] = {8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 512, 1024, 1 << 30};
-    bool found = false;
-    for (int32_t v : ok) found |= v == heavy_degree;
-    if (!found) SS_FAIL(ctx, SS_ERR_INVALID, "heavy_degree must be one of 8,16,24,32,48,64,96,128,192,256,512,1024,2^30");
+    bool hub_ok = false, heavy_ok = false;
+    for (int32_t v : SS_THRESHOLDS) {
+        hub_ok |= v == hub_degree;
+        heavy_ok |= v == heavy_degree;
+    }
+    if (!hub_ok || !heavy_ok)
+        SS_FAIL(ctx, SS_ERR_INVALID, "degree thresholds must be one of 8,16,24,32,48,64,96,128,192,256,512,"
+                                     "1024,2048,4096,16384,2^30");
+    if (hub_degree < heavy_degree) SS_FAIL(ctx, SS_ERR_INVALID, "hub_degree must be >= heavy_degree");
     if (grid_blocks < 0) SS_FAIL(ctx, SS_ERR_INVALID, "grid_blocks must be >= 0");
     SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->hub_deg = hub_degree;
     ctx->heavy_deg = heavy_degree;
     ctx->apl = apl;
     ctx->grid_blocks = grid_blocks;
@@ -830,7 +850,7 @@ int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals
         SS_HIP(ctx, ss_launch_halo_scatter(r.state[0], r.state[1], r.halo_slots, r.halo_dev[b], n, ctx->stream));
         const int par = (int)((r.tick + k) & 1);
         SS_HIP(ctx, ss_launch_bits(r.state[par], r.ibits[par], w0, r.n_words, r.n_total, r.dev.inf_code, ctx->stream));
-        SS_HIP(ctx, ss_launch_tick(ctx->d_reps, ctx->d_tiles, ctx->n_tiles, (int)k, ctx->apl, ctx->grid, ctx->stream));
+        SS_HIP(ctx, tick_launch(ctx, k));
         // halo_dev[b] is rewritten two ticks later: the copy stream waits for this tick's scatter
         hipEvent_t done;
         SS_HIP(ctx, hipEventCreateWithFlags(&done, hipEventDisableTiming));
@@ -847,6 +867,7 @@ int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals
 int ss_set_counting(ss_ctx *ctx, int32_t on) {
     if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
     SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->counting != (on != 0)) drop_graphs(ctx);  // the counter pointer is a captured argument
     ctx->counting = on != 0;
     return SS_OK;
 }

@@ -35,9 +35,10 @@ struct ss_rep_dev {
     int64_t pad1;
 };
 
-// A tile is a 64-aligned agent range of one replica.  LIGHT tiles map one
-// agent to one lane; HEAVY tiles map one agent to one wavefront.
-enum { SS_TILE_LIGHT = 0, SS_TILE_HEAVY = 1 };
+// A tile is an agent range of one replica.  LIGHT tiles (64-aligned) map
+// agents to lanes; HEAVY tiles give each agent's scan to a wavefront; a HUB
+// tile is one agent whose scan takes the whole block.
+enum { SS_TILE_LIGHT = 0, SS_TILE_HEAVY = 1, SS_TILE_HUB = 2 };
 struct ss_tile {
     int32_t rep;
     int32_t kind;

@@ -12,7 +12,13 @@
 
 // kernel launchers (tick_kernels.hip)
 hipError_t ss_launch_tick(const ss_rep_dev *reps, const ss_tile *tiles, int n_tiles, int k, int apl, int grid,
-                          hipStream_t st);
+                          unsigned long long *counters, hipStream_t st);
+// agents of one HEAVY tile (4 wavefronts x 8 agents, tick_kernels.hip SS_HPW)
+#define SS_HEAVY_TILE 32
+// degree thresholds that have a precomputed prefix length per replica
+#define SS_N_THRESHOLDS 16
+static const int32_t SS_THRESHOLDS[SS_N_THRESHOLDS] = {8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 512,
+                                                        1024, 2048, 4096, 16384, 1 << 30};
 hipError_t ss_occupancy_tick(int apl, int *blocks_per_cu);
 hipError_t ss_launch_bits(const int8_t *st, uint64_t *bits, int64_t w0, int64_t w1, int64_t n_total, int8_t inf,
                           hipStream_t s);
@@ -80,8 +86,9 @@ struct ss_ctx {
     int n_tiles = 0;
     bool tiles_dirty = true;
     // tiling (autotuned by the host library)
+    int hub_deg = 1024;
     int heavy_deg = 64;
-    int apl = 1;
+    int apl = 4;
     int grid_blocks = 0;                        // 0: persistent grid from occupancy
     int use_graph = 1;
     int grid = 0;

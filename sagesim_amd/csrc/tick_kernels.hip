@@ -22,10 +22,14 @@
 
 namespace {
 
-// Everything one tile needs from its replica, as wave-uniform scalars (no
-// arrays: a per-lane index into a local array would send it to scratch).
 // Pointers carry the global address space so the loads are global_load, not flat.
 #define SS_G __attribute__((address_space(1)))
+// Entries of the threshold table cached in LDS; rows of light tiles have
+// fewer neighbours than the heavy threshold, so their lookups stay in LDS.
+#define SS_TLDS 64
+
+// Everything one tile needs from its replica, as wave-uniform scalars (no
+// arrays: a per-lane index into a local array would send it to scratch).
 struct TickView {
     const SS_G int64_t *row_ptr;
     const SS_G int32_t *col_idx;
@@ -44,117 +48,350 @@ struct TickView {
     uint32_t seed_lo, seed_hi;
     uint32_t t_lo, t_hi;
     int64_t n_local;
-    unsigned long long *counters;   // null, or work counters of this launch
 };
 
-// Work done by one lane, flushed to the launch counters when they are on:
-// [0] susceptible agents visited, [1] draws, [2] neighbour scans started,
-// [3] CSR entries read by scans.
+// Work done by one lane over the whole launch, flushed to the launch counters
+// when they are on: [0] susceptible agents visited, [1] draws, [2] neighbour
+// scans started, [3] CSR entries read by scans.
 struct Work {
     uint32_t s_agents = 0, draws = 0, scans = 0, edges = 0;
 };
 
-__device__ __forceinline__ uint32_t draw(const TickView V, int64_t i) {
-    const uint32_t hi = V.rng_hi ? V.rng_hi[i] : 0u;
-    return ss_philox4x32_w0(V.t_lo, V.t_hi, V.rng_lo[i], hi, V.seed_lo, V.seed_hi) & SS_U23_MASK;
+__device__ __forceinline__ uint32_t draw_key(const TickView V, uint32_t klo, uint32_t khi) {
+    return ss_philox4x32_w0(V.t_lo, V.t_hi, klo, khi, V.seed_lo, V.seed_hi) & SS_U23_MASK;
 }
 
-// spontaneous transition of a non-susceptible agent (select chain, no indexing)
-__device__ __forceinline__ int8_t transition(const TickView V, int64_t i, int8_t s, Work &w) {
-    const uint32_t thr = s == 0 ? V.tT0 : s == 1 ? V.tT1 : s == 2 ? V.tT2 : s == 3 ? V.tT3 : 0u;
-    if (thr == 0u) return s;
-    ++w.draws;
-    if (draw(V, i) >= thr) return s;
+__device__ __forceinline__ uint32_t draw(const TickView V, int64_t i) {
+    const uint32_t hi = V.rng_hi ? V.rng_hi[i] : 0u;
+    return draw_key(V, V.rng_lo[i], hi);
+}
+
+// spontaneous-transition threshold and target of a state (select chains)
+__device__ __forceinline__ uint32_t trans_thr(const TickView V, int8_t s) {
+    return s == 0 ? V.tT0 : s == 1 ? V.tT1 : s == 2 ? V.tT2 : s == 3 ? V.tT3 : 0u;
+}
+__device__ __forceinline__ int8_t trans_next(const TickView V, int8_t s) {
     return s == 0 ? V.tN0 : s == 1 ? V.tN1 : s == 2 ? V.tN2 : V.tN3;
 }
 
-__device__ __forceinline__ bool infects(const TickView V, uint32_t m, int32_t k) {
-    return m < V.inf_T[k < V.last ? k : V.last];
+// T[min(k, last)], from LDS for small k
+__device__ __forceinline__ uint32_t thr(const TickView V, const uint32_t *sT, int64_t k) {
+    const int32_t kk = (int32_t)(k < V.last ? k : V.last);
+    return kk < SS_TLDS ? sT[kk] : V.inf_T[kk];
 }
 
-// One agent, one lane.  Returns the new state.
-__device__ __forceinline__ int8_t step_lane(const TickView V, int64_t i, int8_t s, Work &w) {
-    if (s != V.s_code) return transition(V, i, s, w);
-    ++w.s_agents;
-    const int64_t b = V.row_ptr[i], e = V.row_ptr[i + 1];
-    const int64_t deg = e - b;
-    if (deg <= 0) return s;
+__device__ __forceinline__ int8_t transition(const TickView V, int64_t i, int8_t s, Work &w) {
+    const uint32_t t = trans_thr(V, s);
+    if (t == 0u) return s;
     ++w.draws;
-    const uint32_t m = draw(V, i);
-    if (!infects(V, m, (int32_t)(deg < V.last ? deg : V.last))) return s;
-    ++w.scans;
+    return draw(V, i) < t ? trans_next(V, s) : s;
+}
+
+// Scan of one row by one lane, eight gathers in flight per round.  Returns
+// whether the count of infectious neighbours reaches the agent's draw.
+__device__ __forceinline__ bool scan_lane(const TickView V, const uint32_t *sT, int64_t b, int64_t e, uint32_t m,
+                                          int32_t &first, Work &w) {
     const SS_G int32_t *__restrict__ col = V.col_idx;
     const SS_G uint64_t *__restrict__ cb = V.cb;
-    int32_t k = 0, first = -1;
-    int64_t j = b;
-    // four independent gathers in flight before the first test
-    for (; j + 4 <= e; j += 4) {
-        w.edges += 4;
-        const int32_t c0 = col[j], c1 = col[j + 1], c2 = col[j + 2], c3 = col[j + 3];
-        const uint64_t w0 = cb[c0 >> 6], w1 = cb[c1 >> 6], w2 = cb[c2 >> 6], w3 = cb[c3 >> 6];
-        const bool h0 = (w0 >> (c0 & 63)) & 1ull, h1 = (w1 >> (c1 & 63)) & 1ull;
-        const bool h2 = (w2 >> (c2 & 63)) & 1ull, h3 = (w3 >> (c3 & 63)) & 1ull;
-        if (!(h0 | h1 | h2 | h3)) continue;
-        if (h0) { if (first < 0) first = c0; if (infects(V, m, ++k)) goto infected; }
-        if (h1) { if (first < 0) first = c1; if (infects(V, m, ++k)) goto infected; }
-        if (h2) { if (first < 0) first = c2; if (infects(V, m, ++k)) goto infected; }
-        if (h3) { if (first < 0) first = c3; if (infects(V, m, ++k)) goto infected; }
-    }
-    for (; j < e; ++j) {
-        ++w.edges;
-        const int32_t c = col[j];
-        if ((cb[c >> 6] >> (c & 63)) & 1ull) {
-            if (first < 0) first = c;
-            if (infects(V, m, ++k)) goto infected;
-        }
-    }
-    return s;
-infected:
-    if (V.history) ss_hist_append(V.history + i * SS_HIST_W, first);
-    return V.s_target;
-}
-
-// One agent, one wavefront (all 64 lanes call with the same i and s).
-__device__ __forceinline__ int8_t step_wave(const TickView V, int64_t i, int8_t s, int lane, Work &w) {
-    // counts are kept by lane 0 only (all lanes see the same agent)
-    if (s != V.s_code) {
-        Work dummy;
-        return transition(V, i, s, lane == 0 ? w : dummy);
-    }
-    if (lane == 0) ++w.s_agents;
-    const int64_t b = V.row_ptr[i], e = V.row_ptr[i + 1];
-    const int64_t deg = e - b;
-    if (deg <= 0) return s;
-    if (lane == 0) ++w.draws;
-    const uint32_t m = draw(V, i);
-    if (!infects(V, m, (int32_t)(deg < V.last ? deg : V.last))) return s;
-    if (lane == 0) ++w.scans;
-    const SS_G int32_t *__restrict__ col = V.col_idx;
-    int32_t k = 0, first = -1;
-    for (int64_t base = b; base < e; base += 64) {
-        const int64_t j = base + lane;
-        if (lane == 0) w.edges += (uint32_t)(e - base < 64 ? e - base : 64);
-        int32_t c = 0;
-        bool hit = false;
-        if (j < e) {
-            c = col[j];
-            hit = (V.cb[c >> 6] >> (c & 63)) & 1ull;
-        }
-        const uint64_t bal = __ballot(hit);
-        if (bal) {
-            if (first < 0) first = __shfl(c, __ffsll((unsigned long long)bal) - 1);
-            k += __popcll(bal);
-            if (infects(V, m, k)) {
-                if (lane == 0 && V.history) ss_hist_append(V.history + i * SS_HIST_W, first);
-                return V.s_target;
+    int32_t k = 0;
+    for (int64_t j = b; j < e; j += 8) {
+        int32_t c[8];
+        uint64_t wd[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) c[q] = col[j + q < e ? j + q : e - 1];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) wd[q] = cb[c[q] >> 6];
+        w.edges += (uint32_t)(e - j < 8 ? e - j : 8);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (j + q < e && ((wd[q] >> (c[q] & 63)) & 1ull)) {
+                if (first < 0) first = c[q];
+                ++k;
+                if (m < thr(V, sT, k)) return true;
             }
         }
     }
-    return s;
+    return false;
 }
 
-__device__ __forceinline__ void flush_work(const TickView V, const Work &w) {
-    if (!V.counters) return;
+// Scan of one row by one wavefront, 256 entries per round.
+__device__ __forceinline__ bool scan_wave(const TickView V, const uint32_t *sT, int64_t b, int64_t e, uint32_t m,
+                                          int lane, int32_t &first, Work &w) {
+    const SS_G int32_t *__restrict__ col = V.col_idx;
+    int32_t k = 0;
+    for (int64_t base = b; base < e; base += 256) {
+        int32_t c[4];
+        bool h[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t j = base + q * 64 + lane;
+            c[q] = j < e ? col[j] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t j = base + q * 64 + lane;
+            h[q] = j < e && ((V.cb[c[q] >> 6] >> (c[q] & 63)) & 1ull);
+        }
+        if (lane == 0) w.edges += (uint32_t)(e - base < 256 ? e - base : 256);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint64_t bal = __ballot(h[q]);
+            if (bal) {
+                if (first < 0) first = __shfl(c[q], __ffsll((unsigned long long)bal) - 1);
+                k += __popcll(bal);
+                if (m < thr(V, sT, k)) return true;
+            }
+        }
+    }
+    return false;
+}
+
+// Neighbour scans of a light tile, gathered in LDS so that all threads of the
+// block scan concurrently instead of each lane walking its own agents in turn.
+#define SS_LIGHT_MAX (SS_BLOCK * 4)
+struct ScanQueue {
+    int64_t b[SS_LIGHT_MAX];       // first CSR entry
+    int32_t deg[SS_LIGHT_MAX];
+    uint32_t m[SS_LIGHT_MAX];      // the agent's draw
+    uint16_t slot[SS_LIGHT_MAX];   // agent - tile begin
+    uint8_t hit[SS_LIGHT_MAX];     // scan result by slot
+    int n;
+};
+
+// LIGHT tile: A agents per lane (agents g0 + a*64 + lane), loads of all A
+// agents issued phase by phase so one memory round trip serves A agents;
+// rows that need a scan go to the block's scan queue.  Every wave reaches the
+// block barriers, whether or not it has agents.
+template <int A>
+__device__ __forceinline__ void light_tile(const TickView V, const uint32_t *sT, int64_t begin, int64_t end,
+                                           Work &w, ScanQueue &Q) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t g0 = begin + (int64_t)wv * 64 * A;
+    if (tid == 0) Q.n = 0;
+    __syncthreads();
+    int8_t s[A], ns[A];
+    int64_t rb[A], re[A];
+    uint32_t klo[A], khi[A];
+    bool scan[A];
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        const int64_t i = g0 + a * 64 + lane;
+        s[a] = i < end ? V.cur[i] : (int8_t)0;
+    }
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        const int64_t i = g0 + a * 64 + lane;
+        rb[a] = re[a] = 0;
+        klo[a] = khi[a] = 0u;
+        if (i < end && i < V.n_local) {
+            const bool sus = s[a] == V.s_code;
+            if (sus) {
+                rb[a] = V.row_ptr[i];
+                re[a] = V.row_ptr[i + 1];
+            }
+            if (sus || trans_thr(V, s[a]) != 0u) {
+                klo[a] = V.rng_lo[i];
+                if (V.rng_hi) khi[a] = V.rng_hi[i];
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        const int64_t i = g0 + a * 64 + lane;
+        ns[a] = s[a];
+        scan[a] = false;
+        if (i < end && i < V.n_local) {
+            if (s[a] == V.s_code) {
+                ++w.s_agents;
+                const int64_t deg = re[a] - rb[a];
+                if (deg > 0) {
+                    ++w.draws;
+                    const uint32_t m = draw_key(V, klo[a], khi[a]);
+                    if (m < thr(V, sT, deg)) {
+                        scan[a] = true;
+                        ++w.scans;
+                        const int q = atomicAdd(&Q.n, 1);
+                        Q.b[q] = rb[a];
+                        Q.deg[q] = (int32_t)deg;
+                        Q.m[q] = m;
+                        Q.slot[q] = (uint16_t)(i - begin);
+                    }
+                }
+            } else {
+                const uint32_t t = trans_thr(V, s[a]);
+                if (t != 0u) {
+                    ++w.draws;
+                    if (draw_key(V, klo[a], khi[a]) < t) ns[a] = trans_next(V, s[a]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int nq = Q.n;
+    for (int q = tid; q < nq; q += SS_BLOCK) {
+        int32_t first = -1;
+        const int64_t b = Q.b[q];
+        const bool inf = scan_lane(V, sT, b, b + Q.deg[q], Q.m[q], first, w);
+        Q.hit[Q.slot[q]] = inf ? 1 : 0;
+        if (inf && V.history) ss_hist_append(V.history + (begin + Q.slot[q]) * SS_HIST_W, first);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        const int64_t ga = g0 + a * 64;
+        if (ga >= end) break;                        // wave-uniform; no barrier follows
+        const int64_t i = ga + lane;
+        if (scan[a] && Q.hit[i - begin]) ns[a] = V.s_target;
+        if (i < end) V.nxt[i] = ns[a];
+        const uint64_t word = __ballot(i < end && ns[a] == V.inf_code);
+        if (lane == 0) V.nb[ga >> 6] = word;
+    }
+}
+
+// Bits of agents in the heavy region are set and cleared one agent at a time
+// (their words are shared by tiles that run concurrently).
+__device__ __forceinline__ void put_bits(SS_G uint64_t *nb, int64_t word, uint64_t set, uint64_t clr) {
+    if (set) atomicOr((unsigned long long *)&nb[word], (unsigned long long)set);
+    if (clr) atomicAnd((unsigned long long *)&nb[word], ~(unsigned long long)clr);
+}
+
+// HEAVY tile: SS_HPW agents per wavefront.  Lanes 0..SS_HPW-1 load, draw and
+// decide for their agent in parallel; the rows that need a scan are then
+// scanned one after another by the whole wavefront.
+#define SS_HPW 8
+__device__ __forceinline__ void heavy_tile(const TickView V, const uint32_t *sT, int64_t begin, int64_t end,
+                                           Work &w) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t i = begin + (int64_t)wv * SS_HPW + lane;
+    const bool mine = lane < SS_HPW && i < end;
+    int8_t s = 0, ns = 0;
+    int64_t b = 0, e = 0;
+    uint32_t m = 0u;
+    bool need = false;
+    if (mine) {
+        s = V.cur[i];
+        ns = s;
+        if (i < V.n_local) {
+            if (s == V.s_code) {
+                ++w.s_agents;
+                b = V.row_ptr[i];
+                e = V.row_ptr[i + 1];
+                if (e > b) {
+                    ++w.draws;
+                    m = draw(V, i);
+                    need = m < thr(V, sT, e - b);
+                    w.scans += need;
+                }
+            } else {
+                ns = transition(V, i, s, w);
+            }
+        }
+    }
+    uint64_t pend = __ballot(need);
+    while (pend) {
+        const int l = __ffsll((unsigned long long)pend) - 1;
+        pend &= pend - 1;
+        const int64_t bl = ((int64_t)__shfl((int32_t)(b >> 32), l) << 32) | (uint32_t)__shfl((int32_t)b, l);
+        const int64_t el = ((int64_t)__shfl((int32_t)(e >> 32), l) << 32) | (uint32_t)__shfl((int32_t)e, l);
+        const uint32_t ml = (uint32_t)__shfl((int32_t)m, l);
+        int32_t first = -1;
+        const bool inf = scan_wave(V, sT, bl, el, ml, lane, first, w);
+        if (lane == l && inf) {
+            ns = V.s_target;
+            if (V.history) ss_hist_append(V.history + i * SS_HIST_W, first);
+        }
+    }
+    if (mine) V.nxt[i] = ns;
+    const int64_t w0 = begin + (int64_t)wv * SS_HPW;
+    if (w0 < end) {
+        // the SS_HPW agents of a wavefront sit in one bitmap word
+        const uint64_t on = __ballot(mine && ns == V.inf_code), off = __ballot(mine && ns != V.inf_code);
+        if (lane == 0) put_bits(V.nb, w0 >> 6, on << (w0 & 63), off << (w0 & 63));
+    }
+}
+
+// HUB tile: one agent, the whole block.  All 256 lanes count infectious
+// neighbours over rounds of 1024 entries; the round's total decides.
+__device__ __forceinline__ void hub_tile(const TickView V, const uint32_t *sT, int64_t i, Work &w, int *s_cnt,
+                                         long long *s_pos) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int8_t s = V.cur[i];
+    int8_t ns = s;
+    int32_t first = -1;
+    if (i < V.n_local) {
+        if (s == V.s_code) {
+            const int64_t b = V.row_ptr[i], e = V.row_ptr[i + 1];
+            if (tid == 0) ++w.s_agents;
+            if (e > b) {
+                if (tid == 0) ++w.draws;
+                const uint32_t m = draw(V, i);
+                if (m < thr(V, sT, e - b)) {
+                    if (tid == 0) ++w.scans;
+                    int32_t k = 0;
+                    for (int64_t base = b; base < e; base += 4 * SS_BLOCK) {
+                        if (tid == 0) { *s_cnt = 0; *s_pos = INT64_MAX; }
+                        __syncthreads();
+                        int32_t c[4];
+                        bool h[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int64_t j = base + q * SS_BLOCK + tid;
+                            c[q] = j < e ? V.col_idx[j] : 0;
+                        }
+                        int cnt = 0;
+                        long long pos = INT64_MAX;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int64_t j = base + q * SS_BLOCK + tid;
+                            h[q] = j < e && ((V.cb[c[q] >> 6] >> (c[q] & 63)) & 1ull);
+                            if (h[q]) {
+                                ++cnt;
+                                if (j < pos) pos = j;
+                            }
+                        }
+                        const uint64_t any = __ballot(cnt > 0);
+                        if (any) {
+#pragma unroll
+                            for (int off = 32; off > 0; off >>= 1) {
+                                cnt += __shfl_down(cnt, off);
+                                const long long o = __shfl_down(pos, off);
+                                pos = o < pos ? o : pos;
+                            }
+                            if (lane == 0) {
+                                atomicAdd(s_cnt, cnt);
+                                atomicMin((unsigned long long *)s_pos, (unsigned long long)pos);
+                            }
+                        }
+                        if (tid == 0) w.edges += (uint32_t)(e - base < 4 * SS_BLOCK ? e - base : 4 * SS_BLOCK);
+                        __syncthreads();
+                        const int32_t got = *s_cnt;
+                        const long long p = *s_pos;
+                        __syncthreads();
+                        if (got > 0) {
+                            if (first < 0) first = V.col_idx[p];
+                            k += got;
+                            if (m < thr(V, sT, k)) {
+                                ns = V.s_target;
+                                break;
+                            }
+                        }
+                    }
+                }
+            }
+        } else if (tid == 0) {
+            ns = transition(V, i, s, w);
+        }
+    }
+    if (tid == 0) {
+        V.nxt[i] = ns;
+        if (ns == V.s_target && s == V.s_code && i < V.n_local && V.history)
+            ss_hist_append(V.history + i * SS_HIST_W, first);
+        const uint64_t bit = 1ull << (i & 63);
+        put_bits(V.nb, i >> 6, ns == V.inf_code ? bit : 0ull, ns == V.inf_code ? 0ull : bit);
+    }
+}
+
+__device__ __forceinline__ void flush_work(unsigned long long *counters, const Work &w) {
     uint32_t a = w.s_agents, d = w.draws, sc = w.scans, ed = w.edges;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -164,99 +401,80 @@ __device__ __forceinline__ void flush_work(const TickView V, const Work &w) {
         ed += __shfl_down(ed, off);
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&V.counters[0], (unsigned long long)a);
-        atomicAdd(&V.counters[1], (unsigned long long)d);
-        atomicAdd(&V.counters[2], (unsigned long long)sc);
-        atomicAdd(&V.counters[3], (unsigned long long)ed);
+        atomicAdd(&counters[0], (unsigned long long)a);
+        atomicAdd(&counters[1], (unsigned long long)d);
+        atomicAdd(&counters[2], (unsigned long long)sc);
+        atomicAdd(&counters[3], (unsigned long long)ed);
     }
-}
-
-template <int APL>
-__device__ __forceinline__ void light_tile(const TickView V, int64_t begin, int64_t end) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    Work w;
-#pragma unroll
-    for (int a = 0; a < APL; ++a) {
-        const int64_t g0 = begin + (int64_t)(wv * APL + a) * 64;
-        if (g0 >= end) break;                        // wave-uniform
-        const int64_t i = g0 + lane;
-        bool inf = false;
-        if (i < end) {
-            const int8_t s = V.cur[i];
-            const int8_t ns = i < V.n_local ? step_lane(V, i, s, w) : s;
-            V.nxt[i] = ns;
-            inf = ns == V.inf_code;
-        }
-        const uint64_t word = __ballot(inf);
-        if (lane == 0) V.nb[g0 >> 6] = word;
-    }
-    flush_work(V, w);
-}
-
-__device__ __forceinline__ void heavy_tile(const TickView V, int64_t begin, int64_t end,
-                                           unsigned long long *sword) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    Work w;
-    if (threadIdx.x == 0) *sword = 0ull;
-    __syncthreads();
-    for (int q = wv; q < 64; q += nw) {
-        const int64_t i = begin + q;
-        if (i >= end) break;                         // wave-uniform
-        const int8_t s = V.cur[i];
-        const int8_t ns = i < V.n_local ? step_wave(V, i, s, lane, w) : s;
-        if (lane == 0) {
-            V.nxt[i] = ns;
-            if (ns == V.inf_code) atomicOr(sword, 1ull << q);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) V.nb[begin >> 6] = *sword;
-    flush_work(V, w);
-    __syncthreads();
 }
 
 // Launch step k of a tick loop: every live replica r runs tick reps[r].tick0 + k.
-template <int APL>
+// Blocks stride over the tile list (hub tiles first, then heavy, then light),
+// so every block gets a share of the expensive tiles; no queue, no atomics.
+template <int A>
 __global__ __launch_bounds__(SS_BLOCK) void tick_kernel(const ss_rep_dev *__restrict__ reps,
-                                                        const ss_tile *__restrict__ tiles, int n_tiles,
-                                                        int k) {
-    __shared__ unsigned long long sword;
+                                                        const ss_tile *__restrict__ tiles, int n_tiles, int k,
+                                                        unsigned long long *counters) {
+    __shared__ uint32_t sT[SS_TLDS];
+    __shared__ int s_cnt;
+    __shared__ long long s_pos;
+    __shared__ ScanQueue Q;
+    Work w;
+    int cached = -1;
+    bool live = false;
+    TickView V;
+    ss_tile nt{};
+    if ((int)blockIdx.x < n_tiles) nt = tiles[blockIdx.x];
     for (int tix = blockIdx.x; tix < n_tiles; tix += gridDim.x) {
-        const ss_tile *Tp = tiles + tix;
-        const int32_t rep = Tp->rep, kind = Tp->kind;
-        const int64_t begin = Tp->begin, end = Tp->end;
-        const ss_rep_dev *R = reps + rep;
-        const int64_t t = R->tick0 + k;
-        if (t >= R->tick_end) continue;              // block-uniform
-        const bool odd = (t & 1) != 0;
-        TickView V;
-        V.row_ptr = (const SS_G int64_t *)R->row_ptr;
-        V.col_idx = (const SS_G int32_t *)R->col_idx;
-        V.rng_lo = (const SS_G uint32_t *)R->rng_lo;
-        V.rng_hi = (const SS_G uint32_t *)R->rng_hi;
-        V.cur = (const SS_G int8_t *)(odd ? R->state[1] : R->state[0]);
-        V.nxt = (SS_G int8_t *)(odd ? R->state[0] : R->state[1]);
-        V.cb = (const SS_G uint64_t *)(odd ? R->ibits[1] : R->ibits[0]);
-        V.nb = (SS_G uint64_t *)(odd ? R->ibits[0] : R->ibits[1]);
-        V.history = R->history;
-        V.inf_T = (const SS_G uint32_t *)R->inf_T;
-        V.last = R->inf_T_last;
-        V.tT0 = R->trans_T[0]; V.tT1 = R->trans_T[1]; V.tT2 = R->trans_T[2]; V.tT3 = R->trans_T[3];
-        V.tN0 = R->trans_next[0]; V.tN1 = R->trans_next[1]; V.tN2 = R->trans_next[2]; V.tN3 = R->trans_next[3];
-        V.s_code = R->s_code;
-        V.inf_code = R->inf_code;
-        V.s_target = R->s_target;
-        V.seed_lo = R->seed_lo;
-        V.seed_hi = R->seed_hi;
-        V.t_lo = (uint32_t)t;
-        V.t_hi = (uint32_t)((uint64_t)t >> 32);
-        V.n_local = R->n_local;
-        V.counters = R->counters;
-        if (kind == SS_TILE_HEAVY)
-            heavy_tile(V, begin, end, &sword);
+        // the next tile's descriptor is loaded while this one runs
+        const int32_t rep = nt.rep, kind = nt.kind;
+        const int64_t begin = nt.begin, end = nt.end;
+        if (tix + (int)gridDim.x < n_tiles) nt = tiles[tix + gridDim.x];
+        if (rep != cached) {                         // block-uniform
+            const ss_rep_dev *R = reps + rep;
+            const int64_t t = R->tick0 + k;
+            live = t < R->tick_end;
+            if (live) {
+                const int32_t last = R->inf_T_last;
+                __syncthreads();                     // the previous replica's table is no longer read
+                if (threadIdx.x < SS_TLDS)
+                    sT[threadIdx.x] = R->inf_T[(int32_t)threadIdx.x < last ? (int32_t)threadIdx.x : last];
+                __syncthreads();
+                const bool odd = (t & 1) != 0;
+                V.row_ptr = (const SS_G int64_t *)R->row_ptr;
+                V.col_idx = (const SS_G int32_t *)R->col_idx;
+                V.rng_lo = (const SS_G uint32_t *)R->rng_lo;
+                V.rng_hi = (const SS_G uint32_t *)R->rng_hi;
+                V.cur = (const SS_G int8_t *)(odd ? R->state[1] : R->state[0]);
+                V.nxt = (SS_G int8_t *)(odd ? R->state[0] : R->state[1]);
+                V.cb = (const SS_G uint64_t *)(odd ? R->ibits[1] : R->ibits[0]);
+                V.nb = (SS_G uint64_t *)(odd ? R->ibits[0] : R->ibits[1]);
+                V.history = R->history;
+                V.inf_T = (const SS_G uint32_t *)R->inf_T;
+                V.last = last;
+                V.tT0 = R->trans_T[0]; V.tT1 = R->trans_T[1]; V.tT2 = R->trans_T[2]; V.tT3 = R->trans_T[3];
+                V.tN0 = R->trans_next[0]; V.tN1 = R->trans_next[1];
+                V.tN2 = R->trans_next[2]; V.tN3 = R->trans_next[3];
+                V.s_code = R->s_code;
+                V.inf_code = R->inf_code;
+                V.s_target = R->s_target;
+                V.seed_lo = R->seed_lo;
+                V.seed_hi = R->seed_hi;
+                V.t_lo = (uint32_t)t;
+                V.t_hi = (uint32_t)((uint64_t)t >> 32);
+                V.n_local = R->n_local;
+            }
+            cached = rep;
+        }
+        if (!live) continue;                         // replica finished: block-uniform
+        if (kind == SS_TILE_LIGHT)
+            light_tile<A>(V, sT, begin, end, w, Q);
+        else if (kind == SS_TILE_HEAVY)
+            heavy_tile(V, sT, begin, end, w);
         else
-            light_tile<APL>(V, begin, end);
+            hub_tile(V, sT, begin, w, &s_cnt, &s_pos);
     }
+    if (counters) flush_work(counters, w);
 }
 
 // Rebuild bitmap words [w0, w1) of one state buffer.
@@ -416,14 +634,21 @@ inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b);
 }  // namespace
 
 // ---- launchers (host) ----
-hipError_t ss_launch_tick(const ss_rep_dev *reps, const ss_tile *tiles, int n_tiles, int k, int apl,
-                          int grid, hipStream_t st) {
+hipError_t ss_launch_tick(const ss_rep_dev *reps, const ss_tile *tiles, int n_tiles, int k, int apl, int grid,
+                          unsigned long long *counters, hipStream_t st) {
     if (n_tiles <= 0) return hipSuccess;
     switch (apl) {
-    case 1: hipLaunchKernelGGL(tick_kernel<1>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k); break;
-    case 2: hipLaunchKernelGGL(tick_kernel<2>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k); break;
-    case 4: hipLaunchKernelGGL(tick_kernel<4>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k); break;
-    default: return hipErrorInvalidValue;
+    case 1:
+        hipLaunchKernelGGL(tick_kernel<1>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k, counters);
+        break;
+    case 2:
+        hipLaunchKernelGGL(tick_kernel<2>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k, counters);
+        break;
+    case 4:
+        hipLaunchKernelGGL(tick_kernel<4>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k, counters);
+        break;
+    default:
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -142,9 +142,13 @@ class Engine:
         return out
 
     # -- run --
-    def set_tiling(self, heavy_degree=64, agents_per_lane=1, grid_blocks=0, use_graph=True):
-        self._ck(self._L.ss_set_tiling(self._h, int(heavy_degree), int(agents_per_lane), int(grid_blocks),
-                                       1 if use_graph else 0))
+    def set_tiling(self, heavy_degree=64, agents_per_lane=4, grid_blocks=0, use_graph=True, hub_degree=1024):
+        """Launch shape: rows of degree >= hub_degree get a block, >= heavy_degree a
+        wavefront, the rest agents_per_lane agents per lane (hub_degree is raised
+        to heavy_degree when below it)."""
+        hub_degree = max(int(hub_degree), int(heavy_degree))
+        self._ck(self._L.ss_set_tiling(self._h, hub_degree, int(heavy_degree), int(agents_per_lane),
+                                       int(grid_blocks), 1 if use_graph else 0))
 
     def launch_shape(self):
         nt, g = ctypes.c_int32(), ctypes.c_int32()

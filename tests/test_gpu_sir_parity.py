@@ -51,12 +51,14 @@ def test_er_sir_every_tick(eng):
     rep.remove()
 
 
-@pytest.mark.parametrize("heavy,apl,graph_loop", [(64, 1, True), (8, 2, True), (1 << 30, 4, False), (16, 1, False)])
-def test_ba_launch_shapes(eng, heavy, apl, graph_loop):
+@pytest.mark.parametrize("hub,heavy,apl,graph_loop", [(1024, 64, 1, True), (16, 8, 2, True),
+                                                      (1 << 30, 1 << 30, 4, False), (64, 16, 1, False),
+                                                      (32, 32, 4, True), (8, 8, 1, True)])
+def test_ba_launch_shapes(eng, hub, heavy, apl, graph_loop):
     """BA(50k, 5), 60 ticks, in one captured loop: every launch shape gives the oracle's states."""
     w = workloads.ba_sir(n=50_000, m=5, beta=0.06, ticks=60, seed=3)
-    say("ba launch shape", heavy, apl, graph_loop)
-    eng.set_tiling(heavy, apl, 0, graph_loop)
+    say("ba launch shape", hub, heavy, apl, graph_loop)
+    eng.set_tiling(heavy, apl, 0, graph_loop, hub_degree=hub)
     rep = Replica(eng, w.graph, w.state, w.model, w.seed)
     want, _ = O.sir_run(oracle_model(w), w.graph.row_ptr, w.graph.col_idx, np.arange(w.graph.n),
                         w.state, w.ticks)
@@ -114,14 +116,15 @@ def test_edge_cases(eng):
     cases.append(("chain_beta1", graphs.from_edges(101, np.arange(100), np.arange(1, 101)), 1.0, 1.0))
     g = graphs.erdos_renyi(1000, 0.01, 9)
     cases.append(("er_gamma1", g, 0.2, 1.0))
-    for name, g, beta, gamma in cases:
+    for (name, g, beta, gamma), (hub, heavy, apl) in zip(cases * 2, [(1024, 64, 1)] * len(cases) + [(8, 8, 2)] * len(cases)):
+        eng.set_tiling(heavy, apl, 0, True, hub_degree=hub)
         for init in ("one", "all"):
             st = np.zeros(g.n, np.int8)
             if init == "one":
                 st[0] = 1
             else:
                 st[:] = 1
-            say(name, init)
+            say(name, init, hub, heavy, apl)
             seed = int(rng.integers(1 << 62))
             rep = Replica(eng, g, st, sir_model(beta, gamma), seed)
             want, trace = O.sir_run(O.sir_model(beta, gamma, seed), g.row_ptr, g.col_idx, np.arange(g.n), st, 30,
@@ -131,11 +134,14 @@ def test_edge_cases(eng):
                 got = rep.states()
                 assert first_diff(got, trace[t]) is None, (name, init, t, first_diff(got, trace[t]))
             rep.remove()
+    eng.set_tiling(64, 1, 0, True)
 
 
-def test_history_and_ghosts(eng):
+@pytest.mark.parametrize("hub,heavy", [(1024, 64), (16, 8)])
+def test_history_and_ghosts(eng, hub, heavy):
     """One SBM partition with ghosts refreshed from a halo each tick and a
     last-infector history: states and history identical to the oracle."""
+    eng.set_tiling(heavy, 1, 0, True, hub_degree=hub)
     pw = workloads.sbm_partition(n_blocks=16, block_size=4000, deg_in=12, deg_out=2, n_parts=4, part=1,
                                  ticks=40, seed=17)
     g = pw.graph
@@ -158,6 +164,7 @@ def test_history_and_ghosts(eng):
     assert (hist != pw.history).any()
     assert n_ghost > 0
     rep.remove()
+    eng.set_tiling(64, 1, 0, True)
 
 
 def test_hash_map_against_dict(eng):
