@@ -73,6 +73,7 @@ def main():
     ap.add_argument("--hub", type=int, default=1024, help="degree from which a row gets a whole block")
     ap.add_argument("--heavy", type=int, default=64, help="degree from which a row gets a whole wavefront")
     ap.add_argument("--apl", type=int, default=4, help="agents per lane of the other rows")
+    ap.add_argument("--no-autotune", action="store_true", help="use --hub/--heavy/--apl as given")
     ap.add_argument("--grid", type=int, default=0, help="blocks of the persistent grid (0: from occupancy)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -99,6 +100,15 @@ def main():
     rep = Replica(eng, g, w.state, w.model, w.seed)
     eng.sync()
     t_upload = time.perf_counter() - t0
+    tuned = None
+    if not args.no_autotune:
+        from sagesim_amd.autotune import autotune
+        t0 = time.perf_counter()
+        best, res = autotune(eng, rep.reset, args.ticks)
+        args.hub, args.heavy, args.apl = best
+        tuned = {"best": list(best), "ms": res[best], "candidates": len(res),
+                 "seconds": round(time.perf_counter() - t0, 2)}
+        log(f"autotune: {best} {res[best]:.3f} ms per {args.ticks}-tick loop ({len(res)} shapes)")
     n_tiles, grid = eng.launch_shape()
     log(f"upload {t_upload:.2f}s, tiles {n_tiles}, grid {grid}")
 
@@ -149,7 +159,7 @@ def main():
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": None if traffic is None else round(traffic, 1),
                     "alg_bytes_per_launch": round(per_launch, 1), "launch_us": round(launch_us, 2),
-                    "kernel": "tick_kernel<1>"}
+                    "kernel": f"tick_kernel<{args.apl}>"}
         log(f"algorithmic {per_launch / 1e6:.2f} MB/launch -> {achieved:.1f} GB/s")
 
     cpu = None
@@ -200,8 +210,8 @@ def main():
         "config": {"workload": f"plain SIR, BA(n={g.n}, m={args.m}), beta={w.beta}, gamma={w.gamma}, "
                                f"{args.ticks} ticks, degree-sorted CSR (int64 row_ptr, int32 col_idx)",
                    "agents": g.n, "ticks": args.ticks, "nnz": g.nnz,
-                   "tiling": {"hub_degree": args.hub, "heavy_degree": args.heavy, "agents_per_lane": args.apl, "grid": grid,
-                              "tiles": n_tiles},
+                   "tiling": {"hub_degree": args.hub, "heavy_degree": args.heavy, "agents_per_lane": args.apl,
+                              "grid": grid, "tiles": n_tiles, "autotune": tuned},
                    "graph_gen_s": round(t_gen, 2), "upload_s": round(t_upload, 2),
                    "final_SIR": counts},
         "roofline": roofline,

@@ -112,7 +112,7 @@ int apply_model(ss_ctx *ctx, ss_replica &r, const ss_model *m) {
 }
 
 void free_replica(ss_replica &r) {
-    dfree(r.row_ptr); dfree(r.col_idx); dfree(r.rng_lo); dfree(r.rng_hi);
+    dfree(r.row_ptr); dfree(r.col_idx); dfree(r.rng_lo); dfree(r.rng_hi); dfree(r.deg16);
     dfree(r.state[0]); dfree(r.state[1]); dfree(r.init_state);
     dfree(r.ibits[0]); dfree(r.ibits[1]);
     dfree(r.history); dfree(r.init_history); dfree(r.inf_T);
@@ -350,7 +350,7 @@ int ss_device_info(ss_ctx *ctx, int32_t *num_cus, int64_t *total_mem, int64_t *c
 int64_t ss_replica_footprint(int64_t n_local, int64_t n_ghost, int64_t nnz, int32_t with_history, int64_t max_deg) {
     int64_t n_total = n_local + n_ghost;
     int64_t words = (n_total + 63) / 64;
-    int64_t b = 8 * (n_total + 1) + 4 * nnz + 4 * n_local + 3 * n_total + 2 * 8 * words +
+    int64_t b = 8 * (n_total + 1) + 4 * nnz + 6 * n_local + 3 * n_total + 2 * 8 * words +
                 4 * (std::min<int64_t>(max_deg, 1 << 24) + 1);
     if (with_history) b += 2 * 4 * n_local * SS_HISTORY_WIDTH;
     return b;
@@ -411,6 +411,7 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
     R_HIP(dalloc(ctx, &r, &r.row_ptr, (size_t)nt + 1));
     R_HIP(dalloc(ctx, &r, &r.col_idx, (size_t)nnz));
     R_HIP(dalloc(ctx, &r, &r.rng_lo, (size_t)nl));
+    R_HIP(dalloc(ctx, &r, &r.deg16, (size_t)nl));
     if (need_hi) R_HIP(dalloc(ctx, &r, &r.rng_hi, (size_t)nl));
     for (int b = 0; b < 2; ++b) {
         R_HIP(dalloc(ctx, &r, &r.state[b], (size_t)nt));
@@ -438,6 +439,10 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
             if (need_hi) hi[i] = (uint32_t)((uint64_t)sp->rng_key_id[i] >> 32);
         }
         if (nl) R_HIP(hipMemcpy(r.rng_lo, lo.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice));
+        std::vector<uint16_t> d16((size_t)std::max<int64_t>(nl, 1));
+        for (int64_t i = 0; i < nl; ++i)
+            d16[i] = (uint16_t)std::min<int64_t>(sp->row_ptr[i + 1] - sp->row_ptr[i], 65535);
+        if (nl) R_HIP(hipMemcpy(r.deg16, d16.data(), sizeof(uint16_t) * nl, hipMemcpyHostToDevice));
         if (need_hi) R_HIP(hipMemcpy(r.rng_hi, hi.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice));
     }
     if (nt) R_HIP(hipMemcpy(r.init_state, sp->state, (size_t)nt, hipMemcpyHostToDevice));
@@ -448,6 +453,7 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
     r.dev.col_idx = r.col_idx;
     r.dev.rng_lo = r.rng_lo;
     r.dev.rng_hi = r.rng_hi;
+    r.dev.deg16 = r.deg16;
     r.dev.state[0] = r.state[0];
     r.dev.state[1] = r.state[1];
     r.dev.ibits[0] = r.ibits[0];
@@ -583,7 +589,8 @@ int ss_history_download(ss_ctx *ctx, int32_t rep, int32_t *out, int64_t n_rows) 
 int ss_set_tiling(ss_ctx *ctx, int32_t hub_degree, int32_t heavy_degree, int32_t apl, int32_t grid_blocks,
                   int32_t use_graph) {
     if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
-    if (apl != 1 && apl != 2 && apl != 4) SS_FAIL(ctx, SS_ERR_INVALID, "agents_per_lane must be 1, 2 or 4");
+    if (apl != 1 && apl != 2 && apl != 4 && apl != 8)
+        SS_FAIL(ctx, SS_ERR_INVALID, "agents_per_lane must be 1, 2, 4 or 8");
     bool hub_ok = false, heavy_ok = false;
     for (int32_t v : SS_THRESHOLDS) {
         hub_ok |= v == hub_degree;

@@ -31,7 +31,7 @@ struct ss_rep_dev {
     int64_t tick0;              // tick of launch step k = 0
     int64_t tick_end;           // ticks >= tick_end are not run (replica finished)
     int64_t n_local, n_total;
-    unsigned long long *counters;   // null, or [4] work counters (see tick_kernels.hip)
+    const uint16_t *deg16;      // [n_local] min(degree, 65535); 65535 means "read row_ptr"
     int64_t pad1;
 };
 

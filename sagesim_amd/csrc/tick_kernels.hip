@@ -41,6 +41,7 @@ struct TickView {
     SS_G uint64_t *nb;
     int32_t *history;
     const SS_G uint32_t *inf_T;
+    const SS_G uint16_t *deg16;
     int32_t last;
     uint32_t tT0, tT1, tT2, tT3;
     int8_t tN0, tN1, tN2, tN3;
@@ -148,20 +149,20 @@ __device__ __forceinline__ bool scan_wave(const TickView V, const uint32_t *sT, 
 
 // Neighbour scans of a light tile, gathered in LDS so that all threads of the
 // block scan concurrently instead of each lane walking its own agents in turn.
-#define SS_LIGHT_MAX (SS_BLOCK * 4)
+#define SS_LIGHT_MAX (SS_BLOCK * 8)
 struct ScanQueue {
-    int64_t b[SS_LIGHT_MAX];       // first CSR entry
-    int32_t deg[SS_LIGHT_MAX];
     uint32_t m[SS_LIGHT_MAX];      // the agent's draw
     uint16_t slot[SS_LIGHT_MAX];   // agent - tile begin
     uint8_t hit[SS_LIGHT_MAX];     // scan result by slot
     int n;
 };
 
-// LIGHT tile: A agents per lane (agents g0 + a*64 + lane), loads of all A
-// agents issued phase by phase so one memory round trip serves A agents;
-// rows that need a scan go to the block's scan queue.  Every wave reaches the
-// block barriers, whether or not it has agents.
+// LIGHT tile: A agents per lane (agents g0 + a*64 + lane).  Round trip 1
+// reads state and degree of all A agents, round trip 2 the RNG keys of those
+// that draw; rows whose draw passes the full-degree bound go to the block's
+// scan queue, where each thread takes one row (row_ptr, then its entries,
+// then their bitmap words).  Every wave reaches the block barriers, whether
+// or not it has agents.
 template <int A>
 __device__ __forceinline__ void light_tile(const TickView V, const uint32_t *sT, int64_t begin, int64_t end,
                                            Work &w, ScanQueue &Q) {
@@ -170,49 +171,38 @@ __device__ __forceinline__ void light_tile(const TickView V, const uint32_t *sT,
     if (tid == 0) Q.n = 0;
     __syncthreads();
     int8_t s[A], ns[A];
-    int64_t rb[A], re[A];
-    uint32_t klo[A], khi[A];
-    bool scan[A];
+    uint32_t deg[A], klo[A], khi[A];
 #pragma unroll
     for (int a = 0; a < A; ++a) {
         const int64_t i = g0 + a * 64 + lane;
         s[a] = i < end ? V.cur[i] : (int8_t)0;
+        deg[a] = i < V.n_local && i < end ? V.deg16[i] : 0u;
     }
 #pragma unroll
     for (int a = 0; a < A; ++a) {
         const int64_t i = g0 + a * 64 + lane;
-        rb[a] = re[a] = 0;
         klo[a] = khi[a] = 0u;
-        if (i < end && i < V.n_local) {
-            const bool sus = s[a] == V.s_code;
-            if (sus) {
-                rb[a] = V.row_ptr[i];
-                re[a] = V.row_ptr[i + 1];
-            }
-            if (sus || trans_thr(V, s[a]) != 0u) {
-                klo[a] = V.rng_lo[i];
-                if (V.rng_hi) khi[a] = V.rng_hi[i];
-            }
+        if (i < end && i < V.n_local && ((s[a] == V.s_code && deg[a] > 0) || trans_thr(V, s[a]) != 0u)) {
+            klo[a] = V.rng_lo[i];
+            if (V.rng_hi) khi[a] = V.rng_hi[i];
         }
     }
+    uint32_t scanned = 0;                            // bit a: agent a went to the queue
 #pragma unroll
     for (int a = 0; a < A; ++a) {
         const int64_t i = g0 + a * 64 + lane;
         ns[a] = s[a];
-        scan[a] = false;
         if (i < end && i < V.n_local) {
             if (s[a] == V.s_code) {
                 ++w.s_agents;
-                const int64_t deg = re[a] - rb[a];
-                if (deg > 0) {
+                if (deg[a] > 0) {
                     ++w.draws;
                     const uint32_t m = draw_key(V, klo[a], khi[a]);
-                    if (m < thr(V, sT, deg)) {
-                        scan[a] = true;
+                    // a capped degree (65535) bounds with the last table entry
+                    if (m < thr(V, sT, deg[a] == 65535u ? (int64_t)V.last : (int64_t)deg[a])) {
+                        scanned |= 1u << a;
                         ++w.scans;
                         const int q = atomicAdd(&Q.n, 1);
-                        Q.b[q] = rb[a];
-                        Q.deg[q] = (int32_t)deg;
                         Q.m[q] = m;
                         Q.slot[q] = (uint16_t)(i - begin);
                     }
@@ -230,10 +220,11 @@ __device__ __forceinline__ void light_tile(const TickView V, const uint32_t *sT,
     const int nq = Q.n;
     for (int q = tid; q < nq; q += SS_BLOCK) {
         int32_t first = -1;
-        const int64_t b = Q.b[q];
-        const bool inf = scan_lane(V, sT, b, b + Q.deg[q], Q.m[q], first, w);
+        const int64_t i = begin + Q.slot[q];
+        const int64_t b = V.row_ptr[i], e = V.row_ptr[i + 1];
+        const bool inf = scan_lane(V, sT, b, e, Q.m[q], first, w);
         Q.hit[Q.slot[q]] = inf ? 1 : 0;
-        if (inf && V.history) ss_hist_append(V.history + (begin + Q.slot[q]) * SS_HIST_W, first);
+        if (inf && V.history) ss_hist_append(V.history + i * SS_HIST_W, first);
     }
     __syncthreads();
 #pragma unroll
@@ -241,7 +232,7 @@ __device__ __forceinline__ void light_tile(const TickView V, const uint32_t *sT,
         const int64_t ga = g0 + a * 64;
         if (ga >= end) break;                        // wave-uniform; no barrier follows
         const int64_t i = ga + lane;
-        if (scan[a] && Q.hit[i - begin]) ns[a] = V.s_target;
+        if (((scanned >> a) & 1u) && Q.hit[i - begin]) ns[a] = V.s_target;
         if (i < end) V.nxt[i] = ns[a];
         const uint64_t word = __ballot(i < end && ns[a] == V.inf_code);
         if (lane == 0) V.nb[ga >> 6] = word;
@@ -451,6 +442,7 @@ __global__ __launch_bounds__(SS_BLOCK) void tick_kernel(const ss_rep_dev *__rest
                 V.nb = (SS_G uint64_t *)(odd ? R->ibits[0] : R->ibits[1]);
                 V.history = R->history;
                 V.inf_T = (const SS_G uint32_t *)R->inf_T;
+                V.deg16 = (const SS_G uint16_t *)R->deg16;
                 V.last = last;
                 V.tT0 = R->trans_T[0]; V.tT1 = R->trans_T[1]; V.tT2 = R->trans_T[2]; V.tT3 = R->trans_T[3];
                 V.tN0 = R->trans_next[0]; V.tN1 = R->trans_next[1];
@@ -647,6 +639,9 @@ hipError_t ss_launch_tick(const ss_rep_dev *reps, const ss_tile *tiles, int n_ti
     case 4:
         hipLaunchKernelGGL(tick_kernel<4>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k, counters);
         break;
+    case 8:
+        hipLaunchKernelGGL(tick_kernel<8>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k, counters);
+        break;
     default:
         return hipErrorInvalidValue;
     }
@@ -658,6 +653,7 @@ hipError_t ss_occupancy_tick(int apl, int *blocks_per_cu) {
     case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<1>, SS_BLOCK, 0);
     case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<2>, SS_BLOCK, 0);
     case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<4>, SS_BLOCK, 0);
+    case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<8>, SS_BLOCK, 0);
     default: return hipErrorInvalidValue;
     }
 }

@@ -53,7 +53,8 @@ def test_er_sir_every_tick(eng):
 
 @pytest.mark.parametrize("hub,heavy,apl,graph_loop", [(1024, 64, 1, True), (16, 8, 2, True),
                                                       (1 << 30, 1 << 30, 4, False), (64, 16, 1, False),
-                                                      (32, 32, 4, True), (8, 8, 1, True)])
+                                                      (32, 32, 4, True), (8, 8, 1, True), (1024, 64, 8, True),
+                                                      (1 << 30, 1 << 30, 8, True)])
 def test_ba_launch_shapes(eng, hub, heavy, apl, graph_loop):
     """BA(50k, 5), 60 ticks, in one captured loop: every launch shape gives the oracle's states."""
     w = workloads.ba_sir(n=50_000, m=5, beta=0.06, ticks=60, seed=3)
