@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--heavy", type=int, default=64, help="degree from which a row gets a whole wavefront")
     ap.add_argument("--apl", type=int, default=4, help="agents per lane of the other rows")
     ap.add_argument("--no-autotune", action="store_true", help="use --hub/--heavy/--apl as given")
+    ap.add_argument("--copy-mib", type=int, default=1024,
+                    help="size of the plain-copy reference run (achievable HBM bandwidth; 0 skips it)")
     ap.add_argument("--grid", type=int, default=0, help="blocks of the persistent grid (0: from occupancy)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -95,6 +97,12 @@ def main():
     log(f"graph BA({g.n}, {args.m}): nnz {g.nnz}, max degree {int(g.degrees().max())}, {t_gen:.1f}s")
 
     eng = Engine(0)
+    copy_gbs = None
+    if args.copy_mib > 0:
+        nb = args.copy_mib << 20
+        ms = eng.copy_probe(nb, 5)
+        copy_gbs = 2 * nb / (ms * 1e-3) / 1e9
+        log(f"plain copy of {args.copy_mib} MiB: {ms:.3f} ms, {copy_gbs:.0f} GB/s read+write")
     eng.set_tiling(args.heavy, args.apl, args.grid, True, hub_degree=args.hub)
     t0 = time.perf_counter()
     rep = Replica(eng, g, w.state, w.model, w.seed)
@@ -159,6 +167,7 @@ def main():
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": None if traffic is None else round(traffic, 1),
                     "alg_bytes_per_launch": round(per_launch, 1), "launch_us": round(launch_us, 2),
+                    "plain_copy_GBs": None if copy_gbs is None else round(copy_gbs, 1),
                     "kernel": f"tick_kernel<{args.apl}>"}
         log(f"algorithmic {per_launch / 1e6:.2f} MB/launch -> {achieved:.1f} GB/s")
 

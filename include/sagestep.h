@@ -165,6 +165,10 @@ int ss_read_counters(ss_ctx *ctx, int64_t *out);
 /* census of the live state: out[0..3] agents per state, out[4] sum of the
  * degrees of susceptible agents (int64 out[5]) */
 int ss_census(ss_ctx *ctx, int32_t rep, int64_t *out);
+/* Plain device copy of `bytes` (multiple of 16) repeated `repeats` times:
+ * average milliseconds per copy.  The known-size reference for hardware
+ * traffic counters and for the achievable HBM bandwidth. */
+int ss_copy_probe(ss_ctx *ctx, int64_t bytes, int32_t repeats, float *ms);
 /* HIP events on the context's stream around queued work */
 int ss_timer_start(ss_ctx *ctx);
 int ss_timer_stop(ss_ctx *ctx, float *ms);

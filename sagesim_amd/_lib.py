@@ -97,6 +97,7 @@ def step_lib():
     _proto(L, "ss_set_counting", I, _P, _c_i32)
     _proto(L, "ss_read_counters", I, _P, _P)
     _proto(L, "ss_census", I, _P, _c_i32, _P)
+    _proto(L, "ss_copy_probe", I, _P, _c_i64, _c_i32, ctypes.POINTER(ctypes.c_float))
     _proto(L, "ss_timer_start", I, _P)
     _proto(L, "ss_timer_stop", I, _P, ctypes.POINTER(ctypes.c_float))
     _step = L

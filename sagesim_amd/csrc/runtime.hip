@@ -903,6 +903,27 @@ int ss_census(ss_ctx *ctx, int32_t rep, int64_t *out) {
     return SS_OK;
 }
 
+int ss_copy_probe(ss_ctx *ctx, int64_t bytes, int32_t repeats, float *ms) {
+    if (!ctx || !ms || bytes <= 0 || bytes % 16 || repeats < 1) SS_FAIL(ctx, SS_ERR_INVALID, "bad probe arguments");
+    void *src = nullptr, *dst = nullptr;
+    SS_HIP(ctx, hipMalloc(&src, (size_t)bytes));
+    hipError_t e = hipMalloc(&dst, (size_t)bytes);
+    if (e != hipSuccess) { dfree(src); SS_HIP(ctx, e); }
+    const int grid = ctx->num_cus * 8;
+    e = ss_launch_fill_u32((uint32_t *)src, bytes / 4, 0x9E3779B9u, grid, ctx->stream);
+    if (e == hipSuccess) e = ss_launch_copy(src, dst, bytes, grid, ctx->stream);  // warm
+    if (e == hipSuccess) e = hipEventRecord(ctx->t_start, ctx->stream);
+    for (int r = 0; r < repeats && e == hipSuccess; ++r) e = ss_launch_copy(src, dst, bytes, grid, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->t_stop, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->t_stop);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, ctx->t_start, ctx->t_stop);
+    if (e == hipSuccess) *ms /= (float)repeats;
+    dfree(src);
+    dfree(dst);
+    SS_HIP(ctx, e);
+    return SS_OK;
+}
+
 int ss_timer_start(ss_ctx *ctx) {
     if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
     SS_HIP(ctx, hipEventRecord(ctx->t_start, ctx->stream));

@@ -621,6 +621,18 @@ __global__ void stats_kernel(const int8_t *__restrict__ st, const int64_t *__res
     }
 }
 
+// Plain streaming copy (16 B per lane per step): the known-size reference for
+// calibrating hardware traffic counters and the achievable HBM bandwidth.
+__global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v ^ (uint32_t)i;
+}
+
 inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -729,5 +741,15 @@ hipError_t ss_launch_stats(const int8_t *st, const int64_t *row_ptr, int64_t n_l
     if (n_local <= 0) return hipSuccess;
     int g = (int)std::min<int64_t>(grid, (int64_t)grid_for(n_local, 256));
     hipLaunchKernelGGL(stats_kernel, dim3(g), dim3(256), 0, s, st, row_ptr, n_local, s_code, out);
+    return hipGetLastError();
+}
+
+hipError_t ss_launch_copy(const void *src, void *dst, int64_t bytes, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, s, (const uint4 *)src, (uint4 *)dst, bytes / 16);
+    return hipGetLastError();
+}
+
+hipError_t ss_launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid), dim3(256), 0, s, p, n, v);
     return hipGetLastError();
 }

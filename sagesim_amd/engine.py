@@ -177,6 +177,12 @@ class Engine:
         self._ck(self._L.ss_census(self._h, rep, ptr(out)))
         return out
 
+    def copy_probe(self, nbytes, repeats=5):
+        """Average ms of a plain device copy of nbytes (the known-size traffic reference)."""
+        ms = ctypes.c_float()
+        self._ck(self._L.ss_copy_probe(self._h, int(nbytes), int(repeats), ctypes.byref(ms)))
+        return ms.value
+
     def timer_start(self):
         self._ck(self._L.ss_timer_start(self._h))
 
