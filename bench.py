@@ -81,6 +81,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-account", action="store_true", help="skip the work-counter pass")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "tick_kernel_ba10m_traffic.txt"))
+    ap.add_argument("--ensemble-replicas", type=int, default=256, help="batched-ensemble leg (0 skips it)")
+    ap.add_argument("--ensemble-ticks", type=int, default=300)
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving leg: seconds of arrivals (0 skips)")
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="serving leg: arrivals per second")
+    ap.add_argument("--plain", action="store_true", help="only the plain 10M-agent leg")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -228,8 +233,81 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    rep.remove()
+    if not args.plain and args.ensemble_replicas > 0:
+        out["ensemble"] = ensemble_leg(eng, args)
+    if not args.plain and args.serve_seconds > 0:
+        out["serving"] = serving_leg(eng, args)
     print(json.dumps(out), flush=True)
     eng.close()
+
+
+def ensemble_leg(eng, args):
+    """Batched ensemble: replicas of log-uniform size packed into one launch per
+    tick and one graph replay, against the per-scenario launch loop."""
+    from sagesim_amd.autotune import autotune
+    from sagesim_amd.ensemble import Ensemble, ensemble_scenarios, prepare, run_batched, run_standalone
+    scs = ensemble_scenarios(args.ensemble_replicas, ticks=args.ensemble_ticks)
+    t0 = time.perf_counter()
+    prepared = prepare(scs, threads=host_threads())
+    t_gen = time.perf_counter() - t0
+    total = sum(p.graph.n for p in prepared)
+    t0 = time.perf_counter()
+    ens = Ensemble.build(eng, prepared)
+    eng.sync()
+    t_up = time.perf_counter() - t0
+    log(f"ensemble: {len(prepared)} replicas, {total} agents, built in {t_gen:.1f}s, uploaded in {t_up:.1f}s")
+    cands = [(h, hv, a) for h in (1024, 4096) for hv in (64, 128) for a in (4, 8)]
+    best, res = autotune(eng, ens.reset, args.ensemble_ticks, candidates=cands)
+    t_b = run_batched(eng, ens)
+    batched = [ens.states(i) for i in range(len(prepared))]
+    ens.remove()
+    alone, t_a = run_standalone(eng, prepared)
+    exact = all(np.array_equal(a, b) for a, b in zip(alone, batched))
+    n = len(prepared)
+    log(f"ensemble: batched {t_b * 1e3:.1f} ms, per-scenario loop {t_a * 1e3:.1f} ms, "
+        f"speedup {t_a / t_b:.2f}, bit-exact {exact}")
+    return {"replicas": n, "agents_total": int(total), "ticks": args.ensemble_ticks,
+            "scenarios_per_s": n / t_b, "agent_ticks_per_s": total * args.ensemble_ticks / t_b,
+            "batched_s": t_b, "per_scenario_loop_s": t_a, "per_scenario_scenarios_per_s": n / t_a,
+            "speedup_vs_per_scenario_loop": t_a / t_b, "bitexact_vs_standalone": exact,
+            "tiling": list(best), "graph_build_s": round(t_gen, 2), "upload_s": round(t_up, 2)}
+
+
+def serving_leg(eng, args):
+    """Poisson arrivals, host graph builders, admission against a 200 GB HBM
+    budget, continuous batching into the shared tick launch; a sample of the
+    results is re-run alone and compared."""
+    from sagesim_amd import Replica
+    from sagesim_amd.ensemble import Prepared, run_standalone
+    from sagesim_amd.serving import Server, serving_requests
+    eng.set_tiling(64, 8, 0, True, hub_degree=4096)
+    arrivals = serving_requests(rate=args.serve_rate, duration=args.serve_seconds)
+
+    def make(req):
+        sc = req.scenario
+        return Replica(eng, model=sc.model(), seed=sc.seed, tick_limit=sc.ticks, data=req.data)
+
+    srv = Server(eng, make, budget_bytes=200 << 30, chunk_ticks=10, builders=max(1, host_threads() - 1))
+    stats, done = srv.run(arrivals)
+    srv.close()
+    summ = stats.summary()
+    rng = np.random.default_rng(1)
+    sample = [done[i] for i in rng.choice(len(done), size=min(4, len(done)), replace=False)] if done else []
+    prepared = []
+    for r in sample:
+        sc = r.scenario
+        g = sc.build_graph()
+        prepared.append(Prepared(sc, g, sc.initial_state()))
+    alone, _ = run_standalone(eng, prepared)
+    summ["sample_bitexact_alone_vs_mix"] = all(np.array_equal(a, r.result) for a, r in zip(alone, sample))
+    summ["agent_ticks"] = int(sum(r.scenario.n * r.scenario.ticks for r in done if r.result is not None))
+    summ["rate_per_s"] = args.serve_rate
+    summ["arrival_window_s"] = args.serve_seconds
+    log(f"serving: {summ}")
+    for k in ("p50_latency_s", "p99_latency_s", "max_latency_s", "scenarios_per_s", "span_s"):
+        summ[k] = round(summ[k], 4)
+    return summ
 
 
 if __name__ == "__main__":

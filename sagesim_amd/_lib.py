@@ -126,6 +126,7 @@ def graph_lib():
     _proto(L, "sg_sbm_part", G, _c_i32, _c_i64, _c_dbl, _c_dbl, _c_i32, _c_i32, _c_u64)
     _proto(L, "sg_degree_sort_csr", ctypes.c_int, _c_i64, _P, _P, _P, _P, _P, _P)
     _proto(L, "sg_scramble_ids", None, _P, _P, _c_i64, _c_u64)
+    _proto(L, "sg_set_num_threads", None, ctypes.c_int)
     _graph = L
     return L
 

@@ -142,6 +142,10 @@ extern "C" {
 
 const char *sg_last_error(void) { return g_err.c_str(); }
 
+// OpenMP threads of the calling thread's later generator calls (lets a pool
+// of builder threads avoid oversubscribing the host).
+void sg_set_num_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
 int64_t sg_n(const sg_graph *g) { return g->n; }
 int64_t sg_n_cols(const sg_graph *g) { return g->n_cols; }
 int64_t sg_nnz(const sg_graph *g) { return (int64_t)g->col_idx.size(); }

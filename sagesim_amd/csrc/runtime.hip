@@ -375,9 +375,8 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
     } else if (nnz != 0) {
         SS_FAIL(ctx, SS_ERR_INVALID, "nnz must be 0 without local rows");
     }
-    for (int64_t e = 0; e < nnz; ++e)
-        if (sp->col_idx[e] < 0 || sp->col_idx[e] >= nt)
-            SS_FAIL(ctx, SS_ERR_RANGE, "col_idx[%lld] = %d outside [0, %lld)", (long long)e, sp->col_idx[e], (long long)nt);
+    // col_idx is range-checked on the device right after its upload (below),
+    // before any kernel indexes with it.
     int rc = check_model(ctx, &sp->model);
     if (rc) return rc;
     bool need_hi = false;
@@ -430,7 +429,21 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
         for (int64_t i = nl + 1; i <= nt; ++i) rp[i] = nnz;  // ghost rows are empty
         R_HIP(hipMemcpy(r.row_ptr, rp.data(), sizeof(int64_t) * rp.size(), hipMemcpyHostToDevice));
     }
-    if (nnz) R_HIP(hipMemcpy(r.col_idx, sp->col_idx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+    if (nnz) {
+        R_HIP(hipMemcpy(r.col_idx, sp->col_idx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+        unsigned long long first_bad = ~0ull;
+        R_HIP(hipMemcpy(ctx->d_stats, &first_bad, sizeof first_bad, hipMemcpyHostToDevice));
+        R_HIP(ss_launch_check_cols(r.col_idx, nnz, nt, ctx->d_stats, ctx->num_cus * 8, ctx->stream));
+        R_HIP(hipStreamSynchronize(ctx->stream));
+        R_HIP(hipMemcpy(&first_bad, ctx->d_stats, sizeof first_bad, hipMemcpyDeviceToHost));
+        if (first_bad != ~0ull) {
+            char b[200];
+            std::snprintf(b, sizeof b, "col_idx[%llu] = %d outside [0, %lld)", first_bad,
+                          sp->col_idx[first_bad], (long long)nt);
+            ctx->err = b;
+            return fail(SS_ERR_RANGE);
+        }
+    }
     {
         std::vector<uint32_t> lo((size_t)std::max<int64_t>(nl, 1)), hi;
         if (need_hi) hi.resize((size_t)nl);

@@ -633,6 +633,16 @@ __global__ void fill_u32_kernel(uint32_t *p, int64_t n, uint32_t v) {
         p[i] = v ^ (uint32_t)i;
 }
 
+// Range check of an uploaded CSR index array: first_bad = smallest e with
+// col[e] outside [0, n_total) (stays ~0 when every entry is in range).
+__global__ void check_cols_kernel(const int32_t *__restrict__ col, int64_t nnz, int64_t n_total,
+                                  unsigned long long *first_bad) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t c = col[e];
+        if (c < 0 || (int64_t)c >= n_total) atomicMin(first_bad, (unsigned long long)e);
+    }
+}
+
 inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -751,5 +761,12 @@ hipError_t ss_launch_copy(const void *src, void *dst, int64_t bytes, int grid, h
 
 hipError_t ss_launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, int grid, hipStream_t s) {
     hipLaunchKernelGGL(fill_u32_kernel, dim3(grid), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t ss_launch_check_cols(const int32_t *col, int64_t nnz, int64_t n_total, unsigned long long *first_bad,
+                                int grid, hipStream_t s) {
+    if (nnz <= 0) return hipSuccess;
+    hipLaunchKernelGGL(check_cols_kernel, dim3(grid), dim3(256), 0, s, col, nnz, n_total, first_bad);
     return hipGetLastError();
 }

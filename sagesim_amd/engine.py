@@ -7,6 +7,7 @@ caller keeps its own agent order; the RNG stays keyed on the caller's ids, so
 results do not depend on the relabelling.
 """
 import ctypes
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -244,6 +245,53 @@ class Engine:
         self._ck(self._L.ss_simulate_halo(self._h, rep, int(ticks), ptr(vals)))
 
 
+@dataclass
+class ReplicaData:
+    """Host arrays of a replica in storage order (degree-sorted), ready to upload.
+    Building one touches no GPU state, so builder threads can prepare them."""
+    graph: CSRGraph
+    perm: np.ndarray
+    inv: np.ndarray
+    state: np.ndarray
+    rng_key_id: np.ndarray
+    history: np.ndarray = None
+
+    @property
+    def n(self):
+        return self.graph.n
+
+    @property
+    def n_total(self):
+        return self.graph.n_cols
+
+
+def _map_hist(h, table, n):
+    h = h.copy()
+    local = (h >= 0) & (h < n)
+    h[local] = table[h[local]]
+    return np.ascontiguousarray(h, np.int32)
+
+
+def prepare_replica(graph: CSRGraph, state, rng_key_id=None, history=None, sort=True):
+    """Degree-sort a graph and carry states, RNG keys and history along."""
+    n = graph.n
+    state = np.asarray(state, np.int8)
+    if state.shape[0] != graph.n_cols:
+        raise ValueError("state must cover local and ghost slots")
+    if rng_key_id is None:
+        rng_key_id = np.arange(n, dtype=np.int64)
+    rng_key_id = np.asarray(rng_key_id, np.int64)
+    if sort:
+        gs, perm, inv = degree_sort(graph)
+    else:
+        gs, perm, inv = graph, np.arange(n, dtype=np.int32), np.arange(n, dtype=np.int32)
+    st = np.concatenate([state[:n][perm], state[n:]])
+    hist = None
+    if history is not None:
+        hist = _map_hist(np.asarray(history, np.int32)[perm], inv, n)
+    return ReplicaData(gs, perm, inv, st, np.ascontiguousarray(rng_key_id[perm]), hist)
+
+
 class Replica:
     """A graph + compartment model placed on an Engine in degree-sorted order.
 
@@ -251,35 +299,18 @@ class Replica:
     and beyond graph.n are ghosts (graph.n_cols - graph.n of them).
     """
 
-    def __init__(self, engine, graph: CSRGraph, state, model, seed, rng_key_id=None,
-                 history=None, tick0=0, tick_limit=0, sort=True):
+    def __init__(self, engine, graph: CSRGraph = None, state=None, model=None, seed=0, rng_key_id=None,
+                 history=None, tick0=0, tick_limit=0, sort=True, data: ReplicaData = None):
+        if data is None:
+            data = prepare_replica(graph, state, rng_key_id, history, sort)
         self.engine = engine
-        self.n = graph.n
-        self.n_ghost = graph.n_cols - graph.n
-        self.n_total = graph.n_cols
-        state = np.asarray(state, np.int8)
-        if state.shape[0] != self.n_total:
-            raise ValueError("state must cover local and ghost slots")
-        if rng_key_id is None:
-            rng_key_id = np.arange(self.n, dtype=np.int64)
-        rng_key_id = np.asarray(rng_key_id, np.int64)
-        if sort:
-            gs, perm, inv = degree_sort(graph)
-        else:
-            gs, perm, inv = graph, np.arange(self.n, dtype=np.int32), np.arange(self.n, dtype=np.int32)
-        self.graph, self.perm, self.inv = gs, perm, inv
-        st = np.concatenate([state[:self.n][perm], state[self.n:]])
-        hist = None
-        if history is not None:
-            hist = self._map_hist(np.asarray(history, np.int32)[perm], inv)
-        self.rep = engine.add_replica(gs.row_ptr, gs.col_idx, rng_key_id[perm], st, model, seed,
-                                      n_ghost=self.n_ghost, history=hist, tick0=tick0, tick_limit=tick_limit)
-
-    def _map_hist(self, h, table):
-        h = h.copy()
-        local = (h >= 0) & (h < self.n)
-        h[local] = table[h[local]]
-        return np.ascontiguousarray(h, np.int32)
+        self.graph, self.perm, self.inv = data.graph, data.perm, data.inv
+        self.n = data.n
+        self.n_total = data.n_total
+        self.n_ghost = self.n_total - self.n
+        self.rep = engine.add_replica(data.graph.row_ptr, data.graph.col_idx, data.rng_key_id, data.state, model,
+                                      seed, n_ghost=self.n_ghost, history=data.history, tick0=tick0,
+                                      tick_limit=tick_limit)
 
     def states(self):
         st = self.engine.state(self.rep, self.n_total)
@@ -296,7 +327,7 @@ class Replica:
         h = self.engine.history(self.rep, self.n)
         out = np.empty_like(h)
         out[self.perm] = h
-        return self._map_hist(out, self.perm)
+        return _map_hist(out, self.perm, self.n)
 
     def tick(self):
         return self.engine.tick(self.rep)

@@ -100,6 +100,11 @@ def degree_sort(g):
     return CSRGraph(new_rp, new_ci[:ci.shape[0]], g.n_cols, g.extra), perm, inv
 
 
+def set_threads(n):
+    """OpenMP threads used by this (calling) thread's generator calls."""
+    graph_lib().sg_set_num_threads(int(n))
+
+
 def scramble_ids(ids, seed):
     """Bijective map of [0, 2^62) used to give agents random-looking int64 ids."""
     ids = np.ascontiguousarray(ids, np.int64)

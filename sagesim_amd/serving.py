@@ -1,0 +1,185 @@
+"""Serving loop: continuous batching of SIR/SEIR scenarios on one GPU.
+
+Requests arrive (Poisson), a host thread pool builds their graphs, and the
+loop admits built requests onto the device while their HBM footprint fits the
+budget.  Every admitted request becomes a replica of the one engine, so all
+in-flight scenarios advance together in the same persistent tick launch; the
+loop runs short chunks of ticks, retires replicas that reached their own tick
+count (download, free), admits newcomers, and repeats.  A scenario's result
+does not depend on what it shared the launch with: each replica draws from
+its own (seed, agent, tick) key and its tiles never touch other replicas.
+"""
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+import math
+import time
+
+import numpy as np
+
+from . import graphs
+from .engine import prepare_replica
+from .ensemble import Scenario
+
+
+def serving_requests(rate=50.0, duration=600.0, lo=50_000, hi=5_000_000, seed=777):
+    """Poisson arrivals at `rate` per second for `duration` seconds.  Each request:
+    SIR or SEIR (E -> I with sigma ~ U(0.1, 0.3)), size log-uniform in [lo, hi],
+    BA (m in {3..6}) or WS (k = 10, rewire 0.1), 100-500 ticks, its own seed;
+    beta ~ U(0.01, 0.08), gamma ~ U(0.05, 0.2)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            break
+        seir = bool(rng.integers(0, 2))
+        ba = bool(rng.integers(0, 2))
+        n = int(math.exp(rng.uniform(math.log(lo), math.log(hi))))
+        sc = Scenario(n=n, graph_kind="ba" if ba else "ws", graph_param=int(rng.integers(3, 7)) if ba else 10,
+                      beta=float(rng.uniform(0.01, 0.08)), gamma=float(rng.uniform(0.05, 0.2)),
+                      sigma=float(rng.uniform(0.1, 0.3)) if seir else 0.0,
+                      seed=int(rng.integers(1, 2**62)), ticks=int(rng.integers(100, 501)))
+        out.append((t, sc))
+    return out
+
+
+def footprint(graph, with_history=False):
+    """Device bytes of a replica (mirrors ss_replica_footprint)."""
+    n = graph.n
+    nt = graph.n_cols
+    words = (nt + 63) // 64
+    b = 8 * (nt + 1) + 4 * graph.nnz + 6 * n + 3 * nt + 16 * words + 4 * (int(graph.degrees().max(initial=0)) + 1)
+    if with_history:
+        b += 2 * 4 * n * 32
+    return b
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float                 # seconds after start
+    scenario: Scenario
+    graph: object = None
+    state: object = None
+    nbytes: int = 0
+    admitted: float = None
+    done: float = None
+    result: np.ndarray = None
+    replica: object = None
+    data: object = None                # ReplicaData prepared by a builder thread
+
+
+@dataclass
+class ServeStats:
+    completed: int = 0
+    submitted: int = 0
+    rejected: int = 0
+    chunks: int = 0
+    span_s: float = 0.0
+    latencies: list = field(default_factory=list)
+    max_inflight: int = 0
+    max_bytes: int = 0
+
+    def summary(self):
+        lat = np.array(self.latencies) if self.latencies else np.zeros(1)
+        return {"completed": self.completed, "submitted": self.submitted, "rejected": self.rejected,
+                "scenarios_per_s": self.completed / self.span_s if self.span_s > 0 else 0.0,
+                "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+                "max_latency_s": float(lat.max()), "chunks": self.chunks, "max_inflight": self.max_inflight,
+                "max_device_bytes": self.max_bytes, "span_s": self.span_s}
+
+
+class Server:
+    """Continuous-batching loop over one engine.
+
+    make_replica(req) -> replica handle with .states(), .tick(), .remove();
+    the engine needs .simulate(ticks) and .sync().  Both are injectable so the
+    admission and retirement logic runs without a GPU."""
+
+    def __init__(self, engine, make_replica, budget_bytes=200 << 30, chunk_ticks=10, builders=8,
+                 keep_results=True, clock=time.perf_counter, builder_threads=1):
+        self.builder_threads = builder_threads
+        self.engine = engine
+        self.make_replica = make_replica
+        self.budget = int(budget_bytes)
+        self.chunk = int(chunk_ticks)
+        self.pool = ThreadPoolExecutor(max_workers=max(1, builders))
+        self.keep = keep_results
+        self.clock = clock
+
+    def _build(self, req):
+        graphs.set_threads(self.builder_threads)
+        sc = req.scenario
+        req.graph = sc.build_graph()
+        req.state = sc.initial_state()
+        req.nbytes = footprint(req.graph)
+        req.data = prepare_replica(req.graph, req.state)      # degree sort off the serving thread
+        return req
+
+    def run(self, arrivals, realtime=True):
+        """Serve (arrival_time, Scenario) pairs; returns (stats, finished requests)."""
+        stats = ServeStats(submitted=len(arrivals))
+        reqs = [Request(i, t, sc) for i, (t, sc) in enumerate(arrivals)]
+        pending = list(reqs)                 # not yet arrived, by arrival time
+        building = []                        # futures, in arrival order
+        ready = []                           # built, waiting for memory (FIFO)
+        inflight = []
+        finished = []
+        used = 0
+        t0 = self.clock()
+        pi = 0
+        while pi < len(pending) or building or ready or inflight:
+            now = self.clock() - t0
+            # arrivals: hand to the builders (not realtime: everything has arrived)
+            while pi < len(pending) and (not realtime or pending[pi].arrival <= now):
+                building.append(self.pool.submit(self._build, pending[pi]))
+                pi += 1
+            # collect built requests in arrival order
+            while building and building[0].done():
+                r = building.pop(0).result()
+                if r.nbytes > self.budget:
+                    stats.rejected += 1          # can never fit: refuse instead of blocking the queue
+                    r.done = self.clock() - t0
+                    finished.append(r)
+                else:
+                    ready.append(r)
+            # admission: FIFO while the footprint fits
+            while ready and used + ready[0].nbytes <= self.budget:
+                r = ready.pop(0)
+                r.replica = self.make_replica(r)
+                r.admitted = self.clock() - t0
+                used += r.nbytes
+                inflight.append(r)
+            stats.max_inflight = max(stats.max_inflight, len(inflight))
+            stats.max_bytes = max(stats.max_bytes, used)
+            if inflight:
+                self.engine.simulate(self.chunk)
+                stats.chunks += 1
+                still = []
+                for r in inflight:
+                    if r.replica.tick() >= r.scenario.ticks:
+                        res = r.replica.states()
+                        r.result = res if self.keep else None
+                        r.replica.remove()
+                        r.replica = None
+                        r.graph = r.state = r.data = None
+                        used -= r.nbytes
+                        r.done = self.clock() - t0
+                        stats.completed += 1
+                        stats.latencies.append(r.done - r.arrival)
+                        finished.append(r)
+                    else:
+                        still.append(r)
+                inflight = still
+            else:
+                # nothing on the device: wait for the next arrival or build
+                if building:
+                    building[0].result()
+                elif pi < len(pending) and realtime:
+                    time.sleep(max(0.0, min(0.01, pending[pi].arrival - (self.clock() - t0))))
+        stats.span_s = self.clock() - t0
+        return stats, finished
+
+    def close(self):
+        self.pool.shutdown(wait=True)

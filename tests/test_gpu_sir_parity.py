@@ -194,3 +194,23 @@ def test_hash_map_against_dict(eng):
     got, _ = eng.map_lookup(rep, ids)
     assert np.array_equal(got, slots)
     eng.remove_replica(rep)
+
+
+def test_bad_csr_rejected(eng):
+    """Out-of-range neighbour slots are refused by the device-side range check,
+    with the offending entry named, and the context stays usable."""
+    from sagesim_amd._lib import NativeError
+    say("bad csr")
+    rp = np.array([0, 2, 3], np.int64)
+    ci = np.array([1, 7, 0], np.int32)          # 7 is outside the 2 slots
+    with pytest.raises(NativeError) as ex:
+        eng.add_replica(rp, ci, np.arange(2), np.zeros(2, np.int8), sir_model(0.1, 0.1), 1)
+    assert "col_idx[1] = 7" in str(ex.value)
+    with pytest.raises(NativeError):
+        eng.add_replica(np.array([0, 2, 1], np.int64), np.array([1, 0], np.int32), np.arange(2),
+                        np.zeros(2, np.int8), sir_model(0.1, 0.1), 1)
+    g = graphs.from_edges(3, [0, 1], [1, 2])
+    rep = Replica(eng, g, np.array([1, 0, 0], np.int8), sir_model(1.0, 0.0), 5)
+    eng.simulate(2)
+    assert rep.states().tolist() == [1, 1, 1]
+    rep.remove()
