@@ -173,6 +173,41 @@ int ss_copy_probe(ss_ctx *ctx, int64_t bytes, int32_t repeats, float *ms);
 int ss_timer_start(ss_ctx *ctx);
 int ss_timer_stop(ss_ctx *ctx, float *ms);
 
+/* ---- general step functions (hipRTC): replace generate_gpu_func and the
+ *      CuPy JIT of the generated module (model.py:2010-2457, :800-865), the
+ *      property tensors of _build_gpu_buffers (model.py:1020-1274) and the
+ *      write-back of the fused kernel (model.py:2368-2408) ----
+ * A program holds n agents (sorted by breed), their CSR, float32 agent ids,
+ * logical ids and breed indices (model.py:1085-1096), float64 globals, up to
+ * 48 properties (element size 1/2/4/8 bytes, `width` elements per agent,
+ * double-buffered or not) and the kernels compiled from `source`, which
+ * includes "sagesim_rt.h" (the device runtime this library embeds). */
+typedef struct ss_prog ss_prog;
+int ss_prog_create(ss_ctx *ctx, int64_t n_agents, ss_prog **out);
+int ss_prog_destroy(ss_prog *prog);
+const char *ss_prog_last_error(const ss_prog *prog);
+int ss_prog_set_csr(ss_prog *prog, const int64_t *row_ptr, const int32_t *col_idx, int64_t nnz);
+/* float32 agent ids, logical ids and breeds as the reference passes them,
+ * plus the logical ids as exact int64 */
+int ss_prog_set_ids(ss_prog *prog, const float *agent_ids, const float *logical_ids, const int64_t *logical_ids64,
+                    const float *breeds);
+int ss_prog_add_prop(ss_prog *prog, int32_t elem_bytes, int64_t width, int32_t double_buffered, const void *init,
+                     int32_t *prop_id);
+int ss_prog_prop_upload(ss_prog *prog, int32_t prop_id, const void *data);
+int ss_prog_prop_download(ss_prog *prog, int32_t prop_id, void *out);
+int ss_prog_set_globals(ss_prog *prog, const double *values, int64_t count);
+int ss_prog_set_seed(ss_prog *prog, int64_t seed);
+/* Compile `source` for gfx950 and load the kernels named in `kernel_names`
+ * ('\n'-separated); the hipRTC log is copied into `log` (may be NULL). */
+int ss_prog_compile(ss_prog *prog, const char *source, const char *kernel_names, char *log, int64_t log_cap);
+/* The launches of one tick, in order: kernel index, first agent, agent count.
+ * Every kernel takes (const SSProgArgs*, long long tick, long long start,
+ * long long count). */
+int ss_prog_set_schedule(ss_prog *prog, int32_t n, const int32_t *kernel, const int64_t *start,
+                         const int64_t *count);
+/* Ticks tick0 .. tick0 + ticks - 1: the schedule, then write-back; waits. */
+int ss_prog_simulate(ss_prog *prog, int64_t tick0, int64_t ticks);
+
 #ifdef __cplusplus
 }
 #endif

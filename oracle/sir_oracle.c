@@ -36,8 +36,9 @@
 /* ------------------------------------------------------------------ */
 /* Reference RNG: Philox-2x32-10, math_utils.py:106-189.               */
 /* counter = ((tick*1000003 + salt) mod 2^32, id mod 2^32), key = seed */
-/* mod 2^32; ten rounds with M = 0xD256D193 and key bump 0x9E3779B9;    */
-/* the output is word 0.                                               */
+/* mod 2^32; ten rounds with M = 3529668003 (0xD26275A3, the decimal    */
+/* math_utils.py:111 multiplies by) and key bump 0x9E3779B9; the output */
+/* is word 0.                                                          */
 /* ------------------------------------------------------------------ */
 uint32_t or_philox2x32_ref(uint32_t key, uint32_t c0, uint32_t c1)
 {

@@ -98,6 +98,19 @@ def step_lib():
     _proto(L, "ss_read_counters", I, _P, _P)
     _proto(L, "ss_census", I, _P, _c_i32, _P)
     _proto(L, "ss_copy_probe", I, _P, _c_i64, _c_i32, ctypes.POINTER(ctypes.c_float))
+    _proto(L, "ss_prog_create", I, _P, _c_i64, PP)
+    _proto(L, "ss_prog_destroy", I, _P)
+    _proto(L, "ss_prog_last_error", ctypes.c_char_p, _P)
+    _proto(L, "ss_prog_set_csr", I, _P, _P, _P, _c_i64)
+    _proto(L, "ss_prog_set_ids", I, _P, _P, _P, _P, _P)
+    _proto(L, "ss_prog_add_prop", I, _P, _c_i32, _c_i64, _c_i32, _P, _P)
+    _proto(L, "ss_prog_prop_upload", I, _P, _c_i32, _P)
+    _proto(L, "ss_prog_prop_download", I, _P, _c_i32, _P)
+    _proto(L, "ss_prog_set_globals", I, _P, _P, _c_i64)
+    _proto(L, "ss_prog_set_seed", I, _P, _c_i64)
+    _proto(L, "ss_prog_compile", I, _P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _c_i64)
+    _proto(L, "ss_prog_set_schedule", I, _P, _c_i32, _P, _P, _P)
+    _proto(L, "ss_prog_simulate", I, _P, _c_i64, _c_i64)
     _proto(L, "ss_timer_start", I, _P)
     _proto(L, "ss_timer_stop", I, _P, ctypes.POINTER(ctypes.c_float))
     _step = L

@@ -26,7 +26,20 @@ def _run(cmd):
     subprocess.check_call(cmd)
 
 
+def embed_step_runtime():
+    """step_rt.h (the device runtime of hipRTC step functions) as a C++ raw
+    string, so libsagestep carries it and needs no header files at run time."""
+    with open(os.path.join(CSRC, "step_rt.h")) as f:
+        body = f.read()
+    assert ")SSRT\"" not in body
+    out = os.path.join(CSRC, "step_rt_embed.inc")
+    with open(out + ".tmp", "w") as f:
+        f.write('R"SSRT(' + body + ')SSRT"\n')
+    os.replace(out + ".tmp", out)
+
+
 def build_step_lib(verbose=True):
+    embed_step_runtime()
     srcs = [os.path.join(CSRC, s) for s in STEP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     out = os.path.join(HERE, "libsagestep.so")
     tmp = out + ".tmp"

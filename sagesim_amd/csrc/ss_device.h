@@ -73,21 +73,6 @@ __device__ __forceinline__ uint32_t ss_draw(const ss_rep_dev &R, int64_t t, int6
                             R.seed_hi) & SS_U23_MASK;
 }
 
-// Reference Philox-2x32-10 word 0 (math_utils.py:106-186): M = 0xD256D193,
-// key bump 0x9E3779B9.
-__host__ __device__ __forceinline__ uint32_t ss_philox2x32_ref(uint32_t key, uint32_t c0, uint32_t c1) {
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        uint64_t p = (uint64_t)c0 * 0xD256D193u;
-        uint32_t n0 = (uint32_t)(p >> 32) ^ key ^ c1;
-        c1 = (uint32_t)p;
-        key += 0x9E3779B9u;
-        c0 = n0;
-    }
-    uint64_t p = (uint64_t)c0 * 0xD256D193u;
-    return (uint32_t)(p >> 32) ^ key ^ c1;
-}
-
 __device__ __forceinline__ bool ss_bit(const uint64_t *bits, int64_t slot) {
     return (bits[slot >> 6] >> (slot & 63)) & 1ull;
 }

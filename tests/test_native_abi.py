@@ -35,6 +35,32 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name)
 
 
+def _declared_arity():
+    """name -> number of parameters of every ss_* declaration in the header."""
+    with open(os.path.join(ROOT, "include", "sagestep.h")) as f:
+        txt = re.sub(r"//[^\n]*", "", f.read())
+    out = {}
+    for m in re.finditer(r"(?:int|int64_t|const char \*)\s*\*?(ss_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", txt):
+        params = m.group(2).strip()
+        out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_ctypes_prototypes_match_header():
+    """Every prototype the Python side declares takes as many arguments as the
+    C declaration (a mismatch would pass garbage through the ABI)."""
+    lib = _lib.step_lib()
+    arity = _declared_arity()
+    assert len(arity) >= 40
+    checked = 0
+    for name, n in arity.items():
+        fn = getattr(lib, name)
+        if fn.argtypes is not None:
+            assert len(fn.argtypes) == n, (name, len(fn.argtypes), n)
+            checked += 1
+    assert checked == len(arity), sorted(set(arity) - {k for k in arity if getattr(lib, k).argtypes is not None})
+
+
 def test_graph_library_exports():
     lib = _lib.graph_lib()
     for name in ("sg_er", "sg_ba", "sg_ws", "sg_sbm_part", "sg_from_edges", "sg_degree_sort_csr",
