@@ -8,9 +8,10 @@ simulation from the seeded initial state (state reset + hipGraph-captured
 tick loop).  Inputs are resident in HBM before the timed region.
 
 Prints one JSON line: agent-ticks/s (whole job), the roofline of the tick
-kernel (algorithmic bytes from the kernel's own work counters over HIP-event
-launch time) and the CPU oracle timed on this host's cores on windows of the
-same run (each window also checked bit-exact against the GPU).
+kernel (state-aware algorithmic bytes of each tick over HIP-event launch
+time, beside the bytes the kernel's own counters say it touched) and the CPU
+oracle timed on this host's cores on windows of the same run (each window
+also checked bit-exact against the GPU).
 """
 import argparse
 import json
@@ -77,6 +78,8 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the plain-copy reference run (achievable HBM bandwidth; 0 skips it)")
     ap.add_argument("--grid", type=int, default=0, help="blocks of the persistent grid (0: from occupancy)")
+    ap.add_argument("--push", type=float, default=0.25,
+                    help="push-direction rule ratio (0: always read every neighbour)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-account", action="store_true", help="skip the work-counter pass")
@@ -109,6 +112,7 @@ def main():
         copy_gbs = 2 * nb / (ms * 1e-3) / 1e9
         log(f"plain copy of {args.copy_mib} MiB: {ms:.3f} ms, {copy_gbs:.0f} GB/s read+write")
     eng.set_tiling(args.heavy, args.apl, args.grid, True, hub_degree=args.hub)
+    eng.set_push(args.push)
     t0 = time.perf_counter()
     rep = Replica(eng, g, w.state, w.model, w.seed)
     eng.sync()
@@ -152,29 +156,48 @@ def main():
     log(f"{args.steps} steps in {wall:.3f}s: {value:.4g} agent-ticks/s, "
         f"tick launch {launch_us:.1f} us, final S/I/R {counts}")
 
-    # algorithmic bytes of the implemented kernel, per tick, from its own counters
+    # Roofline of the tick kernel.  Algorithmic bytes per tick follow the
+    # state-aware model B_S(t) = 2N + sum over susceptible agents of (8 + 5 deg)
+    # (state read + write, a row_ptr pair and 4 B index + 1 B neighbour state
+    # per entry of every susceptible row), taken from the census of each tick
+    # of the same run; the kernel's own counters give the bytes it touched.
     roofline = None
     if not args.no_account:
         eng.set_counting(True)
         rep.reset()
         eng.read_counters()
-        alg = 0.0
-        n_words = (g.n + 63) // 64
+        b_state = touched = 0.0
+        push_ticks = 0
         for _ in range(args.ticks):
+            cen = eng.census(rep.rep)
+            b_state += 2 * g.n + 8 * int(cen[0]) + 5 * int(cen[4])
             eng.simulate(1)
-            s_agents, draws, scans, edges = (int(x) for x in eng.read_counters())
-            alg += 2 * g.n + 8 * n_words + 8 * s_agents + 4 * draws + 5 * edges
+            s_agents, draws, scans, edges, pushed, pt, swept = (int(x) for x in eng.read_counters())
+            push_ticks += pt
+            # per agent of a tile that ran: state read and write, bitmap bit,
+            # and the pushed count on a push tick; per draw: degree and key;
+            # per scan: row_ptr; per scanned entry: index + bitmap; per push:
+            # index + atomic read-modify-write
+            touched += (2.125 + (4 if pt else 0)) * swept + 6 * draws + 8 * scans + 5 * edges + 12 * pushed
         eng.set_counting(False)
-        per_launch = alg / args.ticks
+        per_launch = b_state / args.ticks
         achieved = per_launch / (launch_us * 1e-6) / 1e9
+        full_scan = 2 * g.n + 8 * (g.n + 1) + 5 * g.nnz
         traffic = read_traffic(args.traffic)
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": None if traffic is None else round(traffic, 1),
-                    "alg_bytes_per_launch": round(per_launch, 1), "launch_us": round(launch_us, 2),
+                    "alg_bytes_per_launch": round(per_launch, 1),
+                    "byte_model": "state-aware: 2N + sum over susceptible agents of (8 + 5 degree)",
+                    "touched_bytes_per_launch": round(touched / args.ticks, 1),
+                    "achieved_touched_GBs": round(touched / args.ticks / (launch_us * 1e-6) / 1e9, 2),
+                    "full_scan_bytes_per_launch": full_scan,
+                    "full_scan_equivalent_GBs": round(full_scan / (launch_us * 1e-6) / 1e9, 2),
+                    "launch_us": round(launch_us, 2), "push_ticks": push_ticks,
                     "plain_copy_GBs": None if copy_gbs is None else round(copy_gbs, 1),
                     "kernel": f"tick_kernel<{args.apl}>"}
-        log(f"algorithmic {per_launch / 1e6:.2f} MB/launch -> {achieved:.1f} GB/s")
+        log(f"state-aware {per_launch / 1e6:.2f} MB/launch -> {achieved:.1f} GB/s; touched "
+            f"{touched / args.ticks / 1e6:.2f} MB/launch; push ticks {push_ticks}")
 
     cpu = None
     if not args.no_cpu:
@@ -226,6 +249,7 @@ def main():
                    "agents": g.n, "ticks": args.ticks, "nnz": g.nnz,
                    "tiling": {"hub_degree": args.hub, "heavy_degree": args.heavy, "agents_per_lane": args.apl,
                               "grid": grid, "tiles": n_tiles, "autotune": tuned},
+                   "push_ratio": args.push,
                    "graph_gen_s": round(t_gen, 2), "upload_s": round(t_upload, 2),
                    "final_SIR": counts},
         "roofline": roofline,

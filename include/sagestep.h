@@ -155,12 +155,29 @@ int ss_halo_pack(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n, int8
  * copied on a second stream while the previous tick computes. */
 int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals);
 
+/* ---- step direction: no counterpart in the reference, whose generated
+ *      kernel always reads every neighbour (model.py:2302-2408) ----
+ * A replica whose CSR is symmetric (checked on the device at creation) and
+ * that has no ghost slots may run a tick in the push direction: the tick
+ * before adds, along the row of every agent it leaves infectious, one to a
+ * per-agent count of infectious neighbours and tags the tiles it touched;
+ * the push tick then decides every susceptible agent from its count (no
+ * neighbour scan) and skips untagged tiles.  Tick t writes the counts for
+ * t + 1 when sum(degree of infectious agents at t) <= ratio * (number of
+ * susceptible agents at t); ratio 0 disables it (default 0.25).  Results do
+ * not depend on the direction. */
+int ss_set_push(ss_ctx *ctx, double ratio);
+/* out = 1 when the replica may take the push direction */
+int ss_replica_push_ok(ss_ctx *ctx, int32_t rep, int32_t *out);
+
 /* ---- measurement: replaces the verbose_timing dict of worker_coroutine
  *      (model.py:1351-1640) with device-side events and counters ---- */
 /* Count the work of every tick launch: out[0] susceptible agents visited,
- * out[1] RNG draws, out[2] neighbour scans started, out[3] CSR entries read. */
+ * out[1] RNG draws, out[2] neighbour scans started, out[3] CSR entries read,
+ * out[4] infectious-neighbour counts pushed, out[5] replica-ticks that took
+ * the push direction, out[6] agents of the tiles that ran. */
 int ss_set_counting(ss_ctx *ctx, int32_t on);
-/* read and clear the work counters (int64 out[4]) */
+/* read and clear the work counters (int64 out[7]) */
 int ss_read_counters(ss_ctx *ctx, int64_t *out);
 /* census of the live state: out[0..3] agents per state, out[4] sum of the
  * degrees of susceptible agents (int64 out[5]) */

@@ -116,6 +116,7 @@ void free_replica(ss_replica &r) {
     dfree(r.state[0]); dfree(r.state[1]); dfree(r.init_state);
     dfree(r.ibits[0]); dfree(r.ibits[1]);
     dfree(r.history); dfree(r.init_history); dfree(r.inf_T);
+    dfree(r.cnt[0]); dfree(r.cnt[1]); dfree(r.dec); dfree(r.acc);
     dfree(r.map_keys); dfree(r.map_vals); dfree(r.halo_slots);
     dfree(r.halo_dev[0]); dfree(r.halo_dev[1]);
     r = ss_replica();
@@ -127,16 +128,37 @@ int rebuild_bits(ss_ctx *ctx, ss_replica &r, int par, int64_t w0, int64_t w1) {
     return SS_OK;
 }
 
-void drop_graphs(ss_ctx *ctx) {
-    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
-    ctx->graphs.clear();
-}
-
 int64_t heavy_prefix(const ss_replica &r, int32_t thr) {
     for (size_t q = 0; q < r.heavy_prefix_deg.size(); ++q)
         if (r.heavy_prefix_deg[q] == thr) return r.heavy_prefix_len[q];
     return -1;
 }
+
+// Direction state of the replica's current tick t: statistics of t in slot
+// t % 3 (the other slots cleared), the push decision for t and, when it
+// pushes, the infectious-neighbour counts of t from the infectious bitmap.
+int init_acc(ss_ctx *ctx, ss_replica &r) {
+    SS_HIP(ctx, hipMemsetAsync(r.acc, 0, sizeof(unsigned long long) * SS_ACC_WORDS, ctx->stream));
+    const int slot = (int)((uint64_t)r.tick % 3u);
+    SS_HIP(ctx, ss_launch_acc_init(r.state[r.tick & 1], r.row_ptr, r.n_local, r.dev.s_code, r.dev.inf_code, r.acc,
+                                   slot, ctx->num_cus * 8, ctx->stream));
+    if (r.cnt[0]) {
+        const int par = (int)(r.tick & 1);
+        const uint32_t num = r.dev.push_ok ? ctx->push_num : 0u;
+        for (int b = 0; b < 2; ++b)
+            SS_HIP(ctx, hipMemsetAsync(r.cnt[b], 0, sizeof(uint32_t) * (size_t)r.n_local, ctx->stream));
+        SS_HIP(ctx, ss_launch_count_init(r.row_ptr, r.col_idx, r.ibits[par], r.n_local, heavy_prefix(r, 64),
+                                         r.cnt[par], r.dec + par, r.acc + slot * (SS_ACC_SLOTS * 2), num,
+                                         ctx->num_cus * 8, ctx->stream));
+    }
+    return SS_OK;
+}
+
+void drop_graphs(ss_ctx *ctx) {
+    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
+}
+
 
 // Tiles of every live replica: one HUB tile per row of degree >= hub_deg at the
 // front, HEAVY tiles of SS_HEAVY_TILE agents up to the first 64-aligned slot
@@ -147,6 +169,8 @@ int64_t heavy_prefix(const ss_replica &r, int32_t thr) {
 int build_tiles(ss_ctx *ctx) {
     std::vector<ss_tile> hubs, heavy, light;
     const int64_t cap = (int64_t)SS_BLOCK * ctx->apl;
+    struct Pos { int64_t hub, heavy, light; };
+    std::vector<Pos> pos(ctx->reps.size(), Pos{0, 0, 0});
     for (int32_t ri = 0; ri < (int32_t)ctx->reps.size(); ++ri) {
         ss_replica &r = ctx->reps[ri];
         if (!r.live || r.n_total == 0) continue;
@@ -156,35 +180,63 @@ int build_tiles(ss_ctx *ctx) {
         // hub count rounded up to 8: the 8 agents of a heavy wavefront then
         // always share one bitmap word
         const int64_t nh = std::min((Hh + 7) / 8 * 8, H64);
+        pos[ri] = Pos{(int64_t)hubs.size(), (int64_t)heavy.size(), (int64_t)light.size()};
         for (int64_t b = 0; b < nh; ++b) hubs.push_back({ri, SS_TILE_HUB, b, b + 1, 0});
         for (int64_t b = nh; b < H64; b += SS_HEAVY_TILE)
             heavy.push_back({ri, SS_TILE_HEAVY, b, std::min<int64_t>(b + SS_HEAVY_TILE, H64), 0});
         for (int64_t b = H64; b < r.n_total; b += cap)
             light.push_back({ri, SS_TILE_LIGHT, b, std::min<int64_t>(b + cap, r.n_total), 0});
+        r.dev.n_hub = nh;
+        r.dev.h64 = H64;
+        r.dev.light_cap = (int32_t)cap;
     }
     std::vector<ss_tile> all;
     all.reserve(hubs.size() + heavy.size() + light.size());
     all.insert(all.end(), hubs.begin(), hubs.end());
     all.insert(all.end(), heavy.begin(), heavy.end());
     all.insert(all.end(), light.begin(), light.end());
-    if ((int64_t)all.size() > INT32_MAX) SS_FAIL(ctx, SS_ERR_RANGE, "too many tiles");
+    if ((int64_t)all.size() > INT32_MAX / 2) SS_FAIL(ctx, SS_ERR_RANGE, "too many tiles");
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if ((int64_t)all.size() > ctx->tiles_cap) {
         dfree(ctx->d_tiles);
+        dfree(ctx->d_tact[0]);
+        dfree(ctx->d_tact[1]);
         ctx->d_tiles = nullptr;
+        ctx->d_tact[0] = ctx->d_tact[1] = nullptr;
         int64_t nc = std::max<int64_t>((int64_t)all.size() * 5 / 4, 1024);
         SS_HIP(ctx, dalloc(ctx, nullptr, &ctx->d_tiles, (size_t)nc));
+        SS_HIP(ctx, dalloc(ctx, nullptr, &ctx->d_tact[0], (size_t)nc));
+        SS_HIP(ctx, dalloc(ctx, nullptr, &ctx->d_tact[1], (size_t)nc));
         ctx->tiles_cap = nc;
     }
+    // tile tags start cleared: a replica that was pushing with tile tags runs
+    // every tile at its next tick (its agent tags stay valid)
+    SS_HIP(ctx, hipMemsetAsync(ctx->d_tact[0], 0, (size_t)ctx->tiles_cap, ctx->stream));
+    SS_HIP(ctx, hipMemsetAsync(ctx->d_tact[1], 0, (size_t)ctx->tiles_cap, ctx->stream));
+    for (auto &r : ctx->reps)
+        if (r.live && r.dec) SS_HIP(ctx, ss_launch_dec_demote(r.dec, ctx->stream));
     if (!all.empty())
         SS_HIP(ctx, hipMemcpyAsync(ctx->d_tiles, all.data(), all.size() * sizeof(ss_tile), hipMemcpyHostToDevice,
                                    ctx->stream));
     SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->n_tiles = (int)all.size();
+    const int64_t nhub = (int64_t)hubs.size(), nheavy = (int64_t)heavy.size();
+    for (int32_t ri = 0; ri < (int32_t)ctx->reps.size(); ++ri) {
+        ss_replica &r = ctx->reps[ri];
+        r.dev.tact[0] = ctx->d_tact[0];
+        r.dev.tact[1] = ctx->d_tact[1];
+        r.dev.hub0 = (int32_t)pos[ri].hub;
+        r.dev.heavy0 = (int32_t)(nhub + pos[ri].heavy);
+        r.dev.light0 = (int32_t)(nhub + nheavy + pos[ri].light);
+    }
     int bpc = 0;
     SS_HIP(ctx, ss_occupancy_tick(ctx->apl, &bpc));
     if (bpc < 1) bpc = 1;
     int g = ctx->grid_blocks > 0 ? ctx->grid_blocks : ctx->num_cus * bpc;
     ctx->grid = std::max(1, std::min(g, std::max(ctx->n_tiles, 1)));
+    // the block of a replica's first tile writes its per-tick flags
+    for (auto &r : ctx->reps) r.dev.lead_tile = -1;
+    for (int32_t q = (int32_t)all.size() - 1; q >= 0; --q) ctx->reps[all[q].rep].dev.lead_tile = q;
     drop_graphs(ctx);
     ctx->tiles_dirty = false;
     return SS_OK;
@@ -220,6 +272,7 @@ int push_descriptors(ss_ctx *ctx, int64_t ticks) {
             d.tick0 = r.tick;
             int64_t end = ticks > INT64_MAX - r.tick ? INT64_MAX : r.tick + ticks;
             d.tick_end = std::min(end, r.tick_limit);
+            d.push_num = ctx->push_num;
         } else {
             d.tick0 = 0;
             d.tick_end = 0;
@@ -295,9 +348,9 @@ int ss_create(int device, ss_ctx **out) {
         (e = hipEventCreateWithFlags(&ctx->halo_ev[0], hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&ctx->halo_ev[1], hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreate(&ctx->t_start)) != hipSuccess || (e = hipEventCreate(&ctx->t_stop)) != hipSuccess ||
-        (e = hipMalloc((void **)&ctx->d_work, 4 * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMalloc((void **)&ctx->d_work, SS_WORK_N * sizeof(unsigned long long))) != hipSuccess ||
         (e = hipMalloc((void **)&ctx->d_stats, 8 * sizeof(unsigned long long))) != hipSuccess ||
-        (e = hipMemset(ctx->d_work, 0, 4 * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMemset(ctx->d_work, 0, SS_WORK_N * sizeof(unsigned long long))) != hipSuccess ||
         (e = hipMalloc((void **)&ctx->d_counters, 64 * sizeof(int32_t))) != hipSuccess) {
         delete ctx;
         SS_FAIL(nullptr, SS_ERR_HIP, "context setup: %s", hipGetErrorString(e));
@@ -317,6 +370,8 @@ int ss_destroy(ss_ctx *ctx) {
     for (auto &r : ctx->reps) free_replica(r);
     dfree(ctx->d_reps);
     dfree(ctx->d_tiles);
+    dfree(ctx->d_tact[0]);
+    dfree(ctx->d_tact[1]);
     dfree(ctx->d_counters);
     dfree(ctx->d_work);
     dfree(ctx->d_stats);
@@ -351,7 +406,7 @@ int64_t ss_replica_footprint(int64_t n_local, int64_t n_ghost, int64_t nnz, int3
     int64_t n_total = n_local + n_ghost;
     int64_t words = (n_total + 63) / 64;
     int64_t b = 8 * (n_total + 1) + 4 * nnz + 6 * n_local + 3 * n_total + 2 * 8 * words +
-                4 * (std::min<int64_t>(max_deg, 1 << 24) + 1);
+                4 * (std::min<int64_t>(max_deg, 1 << 24) + 1) + 8 * SS_ACC_WORDS + (n_ghost == 0 ? 8 * n_local + 8 : 0);
     if (with_history) b += 2 * 4 * n_local * SS_HISTORY_WIDTH;
     return b;
 }
@@ -417,6 +472,13 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
         R_HIP(dalloc(ctx, &r, &r.ibits[b], (size_t)r.n_words));
     }
     R_HIP(dalloc(ctx, &r, &r.init_state, (size_t)nt));
+    R_HIP(dalloc(ctx, &r, &r.acc, (size_t)SS_ACC_WORDS));
+    if (ng == 0 && nl > 0) {
+        R_HIP(dalloc(ctx, &r, &r.cnt[0], (size_t)nl));
+        R_HIP(dalloc(ctx, &r, &r.cnt[1], (size_t)nl));
+        R_HIP(dalloc(ctx, &r, &r.dec, (size_t)2));
+        R_HIP(hipMemset(r.dec, 0, 2 * sizeof(uint32_t)));
+    }
     if (sp->history) {
         R_HIP(dalloc(ctx, &r, &r.history, (size_t)nl * SS_HISTORY_WIDTH));
         R_HIP(dalloc(ctx, &r, &r.init_history, (size_t)nl * SS_HISTORY_WIDTH));
@@ -459,6 +521,17 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
         if (need_hi) R_HIP(hipMemcpy(r.rng_hi, hi.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice));
     }
     if (nt) R_HIP(hipMemcpy(r.init_state, sp->state, (size_t)nt, hipMemcpyHostToDevice));
+    // the push direction needs every entry's mirror (a neighbour's row then
+    // lists the agent) and no ghosts (ghost rows are empty)
+    r.symmetric = nnz == 0;
+    if (nnz > 0) {
+        unsigned long long h[4] = {0, 0, 0, 0};
+        R_HIP(hipMemcpy(ctx->d_stats, h, sizeof h, hipMemcpyHostToDevice));
+        R_HIP(ss_launch_symcheck(r.row_ptr, r.col_idx, nl, ctx->d_stats, ctx->num_cus * 8, ctx->stream));
+        R_HIP(hipStreamSynchronize(ctx->stream));
+        R_HIP(hipMemcpy(h, ctx->d_stats, sizeof h, hipMemcpyDeviceToHost));
+        r.symmetric = h[0] == h[2] && h[1] == h[3];
+    }
     if (sp->history)
         R_HIP(hipMemcpy(r.init_history, sp->history, sizeof(int32_t) * nl * SS_HISTORY_WIDTH, hipMemcpyHostToDevice));
     r.dev = ss_rep_dev{};
@@ -472,6 +545,12 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
     r.dev.ibits[0] = r.ibits[0];
     r.dev.ibits[1] = r.ibits[1];
     r.dev.history = r.history;
+    r.dev.cnt[0] = r.cnt[0];
+    r.dev.cnt[1] = r.cnt[1];
+    r.dev.dec = r.dec;
+    r.dev.acc = r.acc;
+    r.dev.push_ok = r.symmetric && r.cnt[0] ? 1u : 0u;
+    r.dev.lead_tile = -1;
     r.dev.seed_lo = (uint32_t)sp->seed;
     r.dev.seed_hi = (uint32_t)(sp->seed >> 32);
     r.dev.n_local = nl;
@@ -517,7 +596,8 @@ int ss_replica_reset(ss_ctx *ctx, int32_t rep) {
                                    hipMemcpyDeviceToDevice, ctx->stream));
     int rc = rebuild_bits(ctx, r, par, 0, r.n_words);
     if (rc) return rc;
-    return rebuild_bits(ctx, r, par ^ 1, 0, r.n_words);
+    if ((rc = rebuild_bits(ctx, r, par ^ 1, 0, r.n_words))) return rc;
+    return init_acc(ctx, r);
 }
 
 int ss_replica_tick(ss_ctx *ctx, int32_t rep, int64_t *tick) {
@@ -539,7 +619,7 @@ int ss_replica_set_tick(ss_ctx *ctx, int32_t rep, int64_t tick, int64_t tick_lim
     }
     r.tick = tick;
     r.tick_limit = tick_limit > 0 ? tick_limit : INT64_MAX;
-    return SS_OK;
+    return init_acc(ctx, r);
 }
 
 int ss_replica_bytes(ss_ctx *ctx, int32_t rep, int64_t *bytes) {
@@ -555,7 +635,8 @@ int ss_set_model(ss_ctx *ctx, int32_t rep, const ss_model *m) {
     int rc = apply_model(ctx, r, m);
     if (rc) return rc;
     const int par = (int)(r.tick & 1);
-    return rebuild_bits(ctx, r, par, 0, r.n_words);
+    if ((rc = rebuild_bits(ctx, r, par, 0, r.n_words))) return rc;
+    return init_acc(ctx, r);
 }
 
 int ss_set_seed(ss_ctx *ctx, int32_t rep, uint64_t seed) {
@@ -586,7 +667,8 @@ int ss_state_upload(ss_ctx *ctx, int32_t rep, const int8_t *in, int64_t n) {
     }
     int rc = rebuild_bits(ctx, r, par, 0, r.n_words);
     if (rc) return rc;
-    return rebuild_bits(ctx, r, par ^ 1, 0, r.n_words);
+    if ((rc = rebuild_bits(ctx, r, par ^ 1, 0, r.n_words))) return rc;
+    return init_acc(ctx, r);
 }
 
 int ss_history_download(ss_ctx *ctx, int32_t rep, int32_t *out, int64_t n_rows) {
@@ -895,10 +977,30 @@ int ss_set_counting(ss_ctx *ctx, int32_t on) {
 int ss_read_counters(ss_ctx *ctx, int64_t *out) {
     if (!ctx || !out) SS_FAIL(ctx, SS_ERR_INVALID, "NULL argument");
     SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    unsigned long long h[4];
+    unsigned long long h[SS_WORK_N];
     SS_HIP(ctx, hipMemcpy(h, ctx->d_work, sizeof h, hipMemcpyDeviceToHost));
     SS_HIP(ctx, hipMemset(ctx->d_work, 0, sizeof h));
-    for (int i = 0; i < 4; ++i) out[i] = (int64_t)h[i];
+    for (int i = 0; i < SS_WORK_N; ++i) out[i] = (int64_t)h[i];
+    return SS_OK;
+}
+
+int ss_set_push(ss_ctx *ctx, double ratio) {
+    if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
+    if (!(ratio >= 0.0) || ratio > 1.0e8) SS_FAIL(ctx, SS_ERR_INVALID, "ratio must be in [0, 1e8]");
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->push_num = (uint32_t)std::lround(ratio * 16.0);
+    // the current tick's decision and tags follow the new rule
+    for (auto &r : ctx->reps)
+        if (r.live) {
+            int rc = init_acc(ctx, r);
+            if (rc) return rc;
+        }
+    return SS_OK;
+}
+
+int ss_replica_push_ok(ss_ctx *ctx, int32_t rep, int32_t *out) {
+    if (!valid_rep(ctx, rep) || !out) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    *out = ctx->reps[rep].dev.push_ok ? 1 : 0;
     return SS_OK;
 }
 

@@ -23,7 +23,7 @@ struct ss_rep_dev {
     int32_t *history;           // [n_local][SS_HIST_W] or null
     const uint32_t *inf_T;      // inf_T[k] = ceil(p_k * 2^23), k = #infectious neighbours
     int32_t inf_T_last;         // last index of inf_T (k is clamped to it)
-    int32_t pad0;
+    uint32_t push_ok;           // CSR is symmetric and there are no ghosts: candidates may be pushed
     uint32_t trans_T[4];        // spontaneous transition thresholds per state (0 = none)
     int8_t trans_next[4];       // ...and target states
     int8_t s_code, inf_code, s_target, n_states;
@@ -32,8 +32,39 @@ struct ss_rep_dev {
     int64_t tick_end;           // ticks >= tick_end are not run (replica finished)
     int64_t n_local, n_total;
     const uint16_t *deg16;      // [n_local] min(degree, 65535); 65535 means "read row_ptr"
-    int64_t pad1;
+    // Push direction.  cnt[t & 1][i] = number of CSR entries of agent i that
+    // are infectious at tick t, pushed by tick t - 1 along the rows of the
+    // agents it leaves infectious (the CSR is symmetric) when it decides to,
+    // and cleared by tick t as it reads them.  dec[t & 1] records the
+    // decision: 0 = pull, 1 = push with tile tags, 2 = push, every tile runs.
+    // Null when !push_ok.
+    uint32_t *cnt[2];
+    uint32_t *dec;
+    // Direction statistics, [tick % 3][slot][2]: [0] = sum of degrees of the
+    // infectious agents, [1] = number of susceptible agents at that tick,
+    // spread over SS_ACC_SLOTS slots to keep atomics apart.  Tick t adds its
+    // changes to the totals of tick t.
+    unsigned long long *acc;
+    uint32_t push_num;          // push when 16 * sum_deg(I) <= push_num * #S (0: never)
+    int32_t lead_tile;          // the tile whose block writes the replica's per-tick flags
+    // Tile of an agent (for the tile tags of the push direction): agents
+    // [0, n_hub) have a HUB tile each from hub0 on, [n_hub, h64) HEAVY tiles of
+    // SS_HEAVY_TILE agents from heavy0 on, the rest LIGHT tiles of light_cap
+    // agents (a power of two) from light0 on.  tact[t & 1][tile] == tag(t): the tile runs at t.
+    uint8_t *tact[2];
+    int64_t n_hub, h64;
+    int32_t hub0, heavy0, light0, light_cap;
 };
+
+#define SS_ACC_SLOTS 16
+#define SS_ACC_WORDS (3 * SS_ACC_SLOTS * 2)
+
+// Tile tag of tick t: never 0 (the cleared value).  A tag equal to that of a
+// tick 510 steps back (same parity) only makes an idle tile run, which
+// changes no result.
+__host__ __device__ __forceinline__ uint8_t ss_front_tag(int64_t t) {
+    return (uint8_t)(1 + (uint64_t)t % 255u);
+}
 
 // A tile is an agent range of one replica.  LIGHT tiles (64-aligned) map
 // agents to lanes; HEAVY tiles give each agent's scan to a wavefront; a HUB

@@ -9,12 +9,17 @@
 #include "ss_device.h"
 
 #define SS_BLOCK 256
+// work counters: S agents visited, draws, scans, CSR entries scanned,
+// counts pushed, replica-ticks taken in the push direction, agents of the
+// tiles that ran
+#define SS_WORK_N 7
 
 // kernel launchers (tick_kernels.hip)
 hipError_t ss_launch_tick(const ss_rep_dev *reps, const ss_tile *tiles, int n_tiles, int k, int apl, int grid,
                           unsigned long long *counters, hipStream_t st);
 // agents of one HEAVY tile (4 wavefronts x 8 agents, tick_kernels.hip SS_HPW)
 #define SS_HEAVY_TILE 32
+#define SS_HEAVY_SHIFT 5
 // degree thresholds that have a precomputed prefix length per replica
 #define SS_N_THRESHOLDS 16
 static const int32_t SS_THRESHOLDS[SS_N_THRESHOLDS] = {8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 512,
@@ -42,6 +47,19 @@ hipError_t ss_launch_check_cols(const int32_t *col, int64_t nnz, int64_t n_total
                                 int grid, hipStream_t s);
 hipError_t ss_launch_stats(const int8_t *st, const int64_t *row_ptr, int64_t n_local, int8_t s_code,
                            unsigned long long *out, int grid, hipStream_t s);
+// push decision of the current tick from its statistics acc_t, then (when
+// pushing) its infectious-neighbour counts (cleared before) from the bitmap
+hipError_t ss_launch_count_init(const int64_t *row_ptr, const int32_t *col_idx, const uint64_t *bits,
+                                int64_t n_local, int64_t heavy, uint32_t *cnt, uint32_t *dec,
+                                const unsigned long long *acc_t, uint32_t push_num, int grid, hipStream_t s);
+// a push decision with tile tags (1) becomes "push, every tile runs" (2)
+hipError_t ss_launch_dec_demote(uint32_t *dec, hipStream_t s);
+// direction statistics of the state buffer st into statistics slot `slot` (zeroed first by the caller)
+hipError_t ss_launch_acc_init(const int8_t *st, const int64_t *row_ptr, int64_t n_local, int8_t s_code, int8_t inf_code,
+                              unsigned long long *acc, int slot, int grid, hipStream_t s);
+// sums of two 64-bit hashes of every entry (i, j) and of its mirror (j, i): out[0..1] and out[2..3]
+hipError_t ss_launch_symcheck(const int64_t *row_ptr, const int32_t *col_idx, int64_t n_rows,
+                              unsigned long long *out, int grid, hipStream_t s);
 
 // One replica's device allocations and host bookkeeping.
 struct ss_replica {
@@ -58,6 +76,10 @@ struct ss_replica {
     uint64_t *ibits[2] = {nullptr, nullptr};
     int32_t *history = nullptr, *init_history = nullptr;
     uint32_t *inf_T = nullptr;
+    uint32_t *cnt[2] = {nullptr, nullptr};     // push-direction infectious-neighbour counts [n_local] by tick parity
+    uint32_t *dec = nullptr;                   // push decisions [2] by tick parity
+    unsigned long long *acc = nullptr;         // direction statistics [SS_ACC_WORDS]
+    bool symmetric = false;                    // every entry (i, j) has its mirror (j, i)
     // id -> slot map for ghosts
     unsigned long long *map_keys = nullptr;
     int32_t *map_vals = nullptr;
@@ -87,6 +109,7 @@ struct ss_ctx {
     ss_rep_dev *h_reps = nullptr;              // pinned host mirror
     int reps_cap = 0;
     ss_tile *d_tiles = nullptr;
+    uint8_t *d_tact[2] = {nullptr, nullptr};   // tile tags of the push direction by tick parity
     int64_t tiles_cap = 0;
     int n_tiles = 0;
     bool tiles_dirty = true;
@@ -99,9 +122,10 @@ struct ss_ctx {
     int grid = 0;
     std::map<int64_t, hipGraphExec_t> graphs;   // by tick count
     int32_t *d_counters = nullptr;              // misc device counters (misses, overflow)
-    unsigned long long *d_work = nullptr;       // [4] tick-kernel work counters
+    unsigned long long *d_work = nullptr;       // [SS_WORK_N] tick-kernel work counters
     unsigned long long *d_stats = nullptr;      // [8] census output
     bool counting = false;
+    uint32_t push_num = 4;                      // direction rule, 16 * ratio (0: pull only); ratio 0.25
     hipEvent_t t_start = nullptr, t_stop = nullptr;
     int8_t *h_halo_pin[2] = {nullptr, nullptr}; // pinned staging for per-tick halo values
     int64_t h_halo_cap = 0;

@@ -14,7 +14,14 @@
 //   * S agents draw first and skip the neighbour scan when even an all-
 //     infectious neighbourhood could not infect them (m >= T[deg]); the scan
 //     stops as soon as the count seen so far infects (T is non-decreasing).
-//     Both shortcuts give the same state as the full count.
+//     Both shortcuts give the same state as the full count;
+//   * push direction (symmetric CSR, no ghosts): when the infectious agents
+//     have few edges compared with the number of susceptible agents, tick t
+//     tags every neighbour of every agent it leaves infectious with tag(t+1)
+//     in a second tag array; at tick t+1 an untagged susceptible agent has no
+//     infectious neighbour, stays susceptible and needs no draw.  The rule
+//     reads statistics of tick t that tick t-1 accumulated, and the decision
+//     is recorded for tick t+1, so one kernel per tick carries both halves.
 #include <algorithm>
 
 #include "ss_device.h"
@@ -22,8 +29,10 @@
 
 namespace {
 
-// Pointers carry the global address space so the loads are global_load, not flat.
+// Pointers carry the global address space so the loads are global_load, not
+// flat; references to the block's LDS record carry the LDS address space.
 #define SS_G __attribute__((address_space(1)))
+#define SS_L __attribute__((address_space(3)))
 // Entries of the threshold table cached in LDS; rows of light tiles have
 // fewer neighbours than the heavy threshold, so their lookups stay in LDS.
 #define SS_TLDS 64
@@ -39,23 +48,141 @@ struct TickView {
     SS_G int8_t *nxt;
     const SS_G uint64_t *cb;
     SS_G uint64_t *nb;
-    int32_t *history;
-    const SS_G uint32_t *inf_T;
     const SS_G uint16_t *deg16;
+    SS_G uint32_t *cnt;          // infectious-neighbour counts of this tick, null when this tick pulls
     int32_t last;
-    uint32_t tT0, tT1, tT2, tT3;
-    int8_t tN0, tN1, tN2, tN3;
     int8_t s_code, inf_code, s_target;
     uint32_t seed_lo, seed_hi;
     uint32_t t_lo, t_hi;
     int64_t n_local;
+    uint8_t tag;                 // tile tag of this tick
+    uint8_t push_next;           // this tick writes the next tick's counts
 };
+
+// The replica's rarely used fields, once per block in LDS (kept out of the
+// scalar registers that the hot fields above occupy).
+struct Cold {
+    uint32_t T[SS_TLDS];         // inf_T[0 .. SS_TLDS)
+    uint32_t tT[4];              // spontaneous-transition thresholds by state
+    int32_t tN[4];               // ...and target states
+    unsigned long long *acc_next; // statistics of the next tick (null: replica not running)
+    const uint8_t *tcur;         // tile tags of this tick, null when every tile runs
+    const uint32_t *inf_T;
+    int32_t *history;
+    uint32_t *pcnt;              // infectious-neighbour counts for the next tick
+    uint8_t *tnext;              // tile tags for the next tick
+    int64_t n_hub, h64;          // agent -> tile mapping of the replica
+    int32_t hub0, heavy0, light0, light_shift;   // light tiles hold 2^light_shift agents
+    uint8_t ptag;                // tile tag of the next tick
+};
+
+typedef SS_L Cold LCold;        // the record as it sits in LDS
+
+// Pushing to the next tick's counts: the fields, copied to registers once
+// per batch of rows.
+struct Pusher {
+    SS_G uint32_t *pcnt;
+    SS_G uint8_t *tnext;
+    int64_t n_hub, h64;
+    int32_t hub0, heavy0, light0, light_shift;
+    uint8_t ptag;
+    __device__ __forceinline__ explicit Pusher(const LCold *C)
+        : pcnt((SS_G uint32_t *)C->pcnt), tnext((SS_G uint8_t *)C->tnext), n_hub(C->n_hub), h64(C->h64),
+          hub0(C->hub0), heavy0(C->heavy0), light0(C->light0), light_shift(C->light_shift), ptag(C->ptag) {}
+    // tile of agent v (see ss_rep_dev)
+    __device__ __forceinline__ int64_t tile_of(int64_t v) const {
+        if (v < n_hub) return hub0 + v;
+        if (v < h64) return heavy0 + ((v - n_hub) >> SS_HEAVY_SHIFT);
+        return light0 + ((v - h64) >> light_shift);
+    }
+    // one more infectious neighbour for agent c next tick; its tile runs then
+    __device__ __forceinline__ void one(int32_t c) const {
+        __hip_atomic_fetch_add(pcnt + c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tnext[tile_of(c)] = ptag;
+    }
+    // entries [b, e) of col, by `width` threads from `me` on, four loads in
+    // flight per thread
+    __device__ __forceinline__ void range(const SS_G int32_t *col, int64_t b, int64_t e, int me, int width) const {
+        for (int64_t j = b + me; j < e; j += 4 * (int64_t)width) {
+            int32_t c[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) c[q] = j + q * width < e ? col[j + q * width] : -1;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (c[q] >= 0) one(c[q]);
+        }
+    }
+};
+
+// Changes of the direction statistics made by this tick, per lane: of the
+// sum of degrees of infectious agents and of the number of susceptible ones.
+struct Dir {
+    int64_t ideg = 0;
+    int32_t sus = 0;
+};
+
+// The change an agent's transition s -> ns makes to the statistics.
+__device__ __forceinline__ void dir_change(Dir &d, const TickView &V, int8_t s, int8_t ns, int64_t deg) {
+    if (ns == s) return;
+    if (s == V.s_code) d.sus -= 1;
+    if (ns == V.inf_code) d.ideg += deg;
+    if (s == V.inf_code) d.ideg -= deg;
+}
+
+// Statistics slot of tick t.
+__device__ __forceinline__ unsigned long long *acc_slot(const ss_rep_dev *R, int64_t t) {
+    return R->acc + ((uint64_t)t % 3u) * (SS_ACC_SLOTS * 2);
+}
+
+// Statistics of tick t: (sum of degrees of infectious agents, #susceptible).
+__device__ __forceinline__ void dir_totals(const ss_rep_dev *R, int64_t t, unsigned long long &d,
+                                           unsigned long long &c) {
+    const unsigned long long *a = acc_slot(R, t);
+    d = 0;
+    c = 0;
+#pragma unroll
+    for (int q = 0; q < SS_ACC_SLOTS; ++q) {
+        d += a[2 * q];
+        c += a[2 * q + 1];
+    }
+}
+
+// Whether the tags for tick t + 1 are to be written while running tick t:
+// the rule applied to the statistics of tick t.
+__device__ __forceinline__ bool push_rule(const ss_rep_dev *R, int64_t t) {
+    if (!R->push_ok || R->push_num == 0u || !R->cnt[0]) return false;
+    unsigned long long d, c;
+    dir_totals(R, t, d, c);
+    return d * 16ull <= (unsigned long long)R->push_num * c;
+}
+
+// Push the rows of the lanes flagged in `rows` (one after another, each by
+// the whole wavefront, lanes striding its entries); b, e are each lane's row.
+__device__ __forceinline__ uint32_t push_rows(const TickView &V, const LCold *C, uint64_t rows, int64_t b,
+                                              int64_t e, int lane) {
+    uint32_t n = 0;
+    if (!rows) return 0;
+    const Pusher P(C);
+    while (rows) {
+        const int l = __ffsll((unsigned long long)rows) - 1;
+        rows &= rows - 1;
+        const int64_t bl = ((int64_t)__shfl((int32_t)(b >> 32), l) << 32) | (uint32_t)__shfl((int32_t)b, l);
+        const int64_t el = ((int64_t)__shfl((int32_t)(e >> 32), l) << 32) | (uint32_t)__shfl((int32_t)e, l);
+        P.range(V.col_idx, bl, el, lane, 64);
+        n += (uint32_t)(el - bl);
+    }
+    return n;
+}
+
+
 
 // Work done by one lane over the whole launch, flushed to the launch counters
 // when they are on: [0] susceptible agents visited, [1] draws, [2] neighbour
-// scans started, [3] CSR entries read by scans.
+// scans started, [3] CSR entries read by scans, [4] counts pushed for the next
+// tick, [6] agents of the tiles that ran ([5] counts replica-ticks that read
+// pushed counts).
 struct Work {
-    uint32_t s_agents = 0, draws = 0, scans = 0, edges = 0;
+    uint32_t s_agents = 0, draws = 0, scans = 0, edges = 0, pushed = 0, swept = 0;
 };
 
 __device__ __forceinline__ uint32_t draw_key(const TickView V, uint32_t klo, uint32_t khi) {
@@ -67,30 +194,30 @@ __device__ __forceinline__ uint32_t draw(const TickView V, int64_t i) {
     return draw_key(V, V.rng_lo[i], hi);
 }
 
-// spontaneous-transition threshold and target of a state (select chains)
-__device__ __forceinline__ uint32_t trans_thr(const TickView V, int8_t s) {
-    return s == 0 ? V.tT0 : s == 1 ? V.tT1 : s == 2 ? V.tT2 : s == 3 ? V.tT3 : 0u;
+// spontaneous-transition threshold (0: none) and target of a state
+__device__ __forceinline__ uint32_t trans_thr(const LCold *C, int8_t s) {
+    return (uint8_t)s < 4u ? C->tT[(uint8_t)s] : 0u;
 }
-__device__ __forceinline__ int8_t trans_next(const TickView V, int8_t s) {
-    return s == 0 ? V.tN0 : s == 1 ? V.tN1 : s == 2 ? V.tN2 : V.tN3;
+__device__ __forceinline__ int8_t trans_next(const LCold *C, int8_t s) {
+    return (int8_t)C->tN[(uint8_t)s & 3u];
 }
 
 // T[min(k, last)], from LDS for small k
-__device__ __forceinline__ uint32_t thr(const TickView V, const uint32_t *sT, int64_t k) {
+__device__ __forceinline__ uint32_t thr(const TickView V, const LCold *C, int64_t k) {
     const int32_t kk = (int32_t)(k < V.last ? k : V.last);
-    return kk < SS_TLDS ? sT[kk] : V.inf_T[kk];
+    return kk < SS_TLDS ? C->T[kk] : ((const SS_G uint32_t *)C->inf_T)[kk];
 }
 
-__device__ __forceinline__ int8_t transition(const TickView V, int64_t i, int8_t s, Work &w) {
-    const uint32_t t = trans_thr(V, s);
+__device__ __forceinline__ int8_t transition(const TickView V, const LCold *C, int64_t i, int8_t s, Work &w) {
+    const uint32_t t = trans_thr(C, s);
     if (t == 0u) return s;
     ++w.draws;
-    return draw(V, i) < t ? trans_next(V, s) : s;
+    return draw(V, i) < t ? trans_next(C, s) : s;
 }
 
 // Scan of one row by one lane, eight gathers in flight per round.  Returns
 // whether the count of infectious neighbours reaches the agent's draw.
-__device__ __forceinline__ bool scan_lane(const TickView V, const uint32_t *sT, int64_t b, int64_t e, uint32_t m,
+__device__ __forceinline__ bool scan_lane(const TickView V, const LCold *C, int64_t b, int64_t e, uint32_t m,
                                           int32_t &first, Work &w) {
     const SS_G int32_t *__restrict__ col = V.col_idx;
     const SS_G uint64_t *__restrict__ cb = V.cb;
@@ -108,7 +235,7 @@ __device__ __forceinline__ bool scan_lane(const TickView V, const uint32_t *sT, 
             if (j + q < e && ((wd[q] >> (c[q] & 63)) & 1ull)) {
                 if (first < 0) first = c[q];
                 ++k;
-                if (m < thr(V, sT, k)) return true;
+                if (m < thr(V, C, k)) return true;
             }
         }
     }
@@ -116,7 +243,7 @@ __device__ __forceinline__ bool scan_lane(const TickView V, const uint32_t *sT, 
 }
 
 // Scan of one row by one wavefront, 256 entries per round.
-__device__ __forceinline__ bool scan_wave(const TickView V, const uint32_t *sT, int64_t b, int64_t e, uint32_t m,
+__device__ __forceinline__ bool scan_wave(const TickView V, const LCold *C, int64_t b, int64_t e, uint32_t m,
                                           int lane, int32_t &first, Work &w) {
     const SS_G int32_t *__restrict__ col = V.col_idx;
     int32_t k = 0;
@@ -140,7 +267,7 @@ __device__ __forceinline__ bool scan_wave(const TickView V, const uint32_t *sT, 
             if (bal) {
                 if (first < 0) first = __shfl(c[q], __ffsll((unsigned long long)bal) - 1);
                 k += __popcll(bal);
-                if (m < thr(V, sT, k)) return true;
+                if (m < thr(V, C, k)) return true;
             }
         }
     }
@@ -158,36 +285,63 @@ struct ScanQueue {
 };
 
 // LIGHT tile: A agents per lane (agents g0 + a*64 + lane).  Round trip 1
-// reads state and degree of all A agents, round trip 2 the RNG keys of those
-// that draw; rows whose draw passes the full-degree bound go to the block's
-// scan queue, where each thread takes one row (row_ptr, then its entries,
-// then their bitmap words).  Every wave reaches the block barriers, whether
+// reads the states (and, on a push tick, the infectious-neighbour counts,
+// which are cleared as they are read), round trip 2 the degree and RNG key of
+// the agents that draw.  Pull tick: rows whose draw passes the full-degree
+// bound go to the block's scan queue, where each thread takes one row
+// (row_ptr, then its entries, then their bitmap words).  Push tick: the count
+// decides at once; only the last-infector history still scans (the queue
+// entry is flagged SS_QHIST).  Every wave reaches the block barriers, whether
 // or not it has agents.
+#define SS_QHIST 0x8000u
 template <int A>
-__device__ __forceinline__ void light_tile(const TickView V, const uint32_t *sT, int64_t begin, int64_t end,
-                                           Work &w, ScanQueue &Q) {
+__device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int64_t begin, int64_t end,
+                                           int64_t tix, Work &w, Dir &dir, ScanQueue &Q) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t g0 = begin + (int64_t)wv * 64 * A;
     if (tid == 0) Q.n = 0;
     __syncthreads();
     int8_t s[A], ns[A];
-    uint32_t deg[A], klo[A], khi[A];
+    uint32_t cand = 0u;                              // bit a: agent a is a susceptible candidate
+    uint32_t key[A], khi[A], kc[A];                  // kc: infectious neighbours (push tick)
+    if (V.cnt) {
 #pragma unroll
-    for (int a = 0; a < A; ++a) {
-        const int64_t i = g0 + a * 64 + lane;
-        s[a] = i < end ? V.cur[i] : (int8_t)0;
-        deg[a] = i < V.n_local && i < end ? V.deg16[i] : 0u;
-    }
+        for (int a = 0; a < A; ++a) {
+            const int64_t i = g0 + a * 64 + lane;
+            s[a] = i < end ? V.cur[i] : (int8_t)0;
+            kc[a] = i < end && i < V.n_local ? V.cnt[i] : 0u;
+        }
 #pragma unroll
-    for (int a = 0; a < A; ++a) {
-        const int64_t i = g0 + a * 64 + lane;
-        klo[a] = khi[a] = 0u;
-        if (i < end && i < V.n_local && ((s[a] == V.s_code && deg[a] > 0) || trans_thr(V, s[a]) != 0u)) {
-            klo[a] = V.rng_lo[i];
-            if (V.rng_hi) khi[a] = V.rng_hi[i];
+        for (int a = 0; a < A; ++a) {
+            const int64_t i = g0 + a * 64 + lane;
+            if (kc[a] != 0u) V.cnt[i] = 0u;           // ready for the tick after next
+            cand |= (s[a] == V.s_code && kc[a] != 0u ? 1u : 0u) << a;
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+            const int64_t i = g0 + a * 64 + lane;
+            s[a] = i < end ? V.cur[i] : (int8_t)0;
+            kc[a] = 0u;
+            cand |= (s[a] == V.s_code ? 1u : 0u) << a;
         }
     }
-    uint32_t scanned = 0;                            // bit a: agent a went to the queue
+    uint32_t deg[A];
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        const int64_t i = g0 + a * 64 + lane;
+        deg[a] = key[a] = khi[a] = 0u;
+        if (i < end && i < V.n_local) {
+            const bool sc = (cand >> a) & 1u;
+            const bool tr = trans_thr(C, s[a]) != 0u;
+            if (sc || tr || s[a] == V.inf_code) deg[a] = V.deg16[i];
+            if (sc || tr) {
+                key[a] = V.rng_lo[i];
+                if (V.rng_hi) khi[a] = V.rng_hi[i];
+            }
+        }
+    }
+    uint32_t scanned = 0;                            // bit a: agent a went to the queue (pull)
 #pragma unroll
     for (int a = 0; a < A; ++a) {
         const int64_t i = g0 + a * 64 + lane;
@@ -195,11 +349,22 @@ __device__ __forceinline__ void light_tile(const TickView V, const uint32_t *sT,
         if (i < end && i < V.n_local) {
             if (s[a] == V.s_code) {
                 ++w.s_agents;
-                if (deg[a] > 0) {
+                if (deg[a] > 0 && ((cand >> a) & 1u)) {
                     ++w.draws;
-                    const uint32_t m = draw_key(V, klo[a], khi[a]);
-                    // a capped degree (65535) bounds with the last table entry
-                    if (m < thr(V, sT, deg[a] == 65535u ? (int64_t)V.last : (int64_t)deg[a])) {
+                    const uint32_t m = draw_key(V, key[a], khi[a]);
+                    if (V.cnt) {
+                        if (m < thr(V, C, (int64_t)kc[a])) {
+                            ns[a] = V.s_target;
+                            dir_change(dir, V, s[a], ns[a], deg[a] == 65535u ? (int64_t)(V.row_ptr[i + 1] - V.row_ptr[i])
+                                                                             : (int64_t)deg[a]);
+                            if (C->history) {
+                                const int q = atomicAdd(&Q.n, 1);
+                                Q.m[q] = 0u;
+                                Q.slot[q] = (uint16_t)((uint32_t)(i - begin) | SS_QHIST);
+                            }
+                        }
+                    } else if (m < thr(V, C, deg[a] == 65535u ? (int64_t)V.last : (int64_t)deg[a])) {
+                        // a capped degree (65535) bounds with the last table entry
                         scanned |= 1u << a;
                         ++w.scans;
                         const int q = atomicAdd(&Q.n, 1);
@@ -208,25 +373,34 @@ __device__ __forceinline__ void light_tile(const TickView V, const uint32_t *sT,
                     }
                 }
             } else {
-                const uint32_t t = trans_thr(V, s[a]);
+                const uint32_t t = trans_thr(C, s[a]);
                 if (t != 0u) {
                     ++w.draws;
-                    if (draw_key(V, klo[a], khi[a]) < t) ns[a] = trans_next(V, s[a]);
+                    if (draw_key(V, key[a], khi[a]) < t) ns[a] = trans_next(C, s[a]);
                 }
+                dir_change(dir, V, s[a], ns[a], deg[a]);
             }
         }
     }
     __syncthreads();
     const int nq = Q.n;
-    for (int q = tid; q < nq; q += SS_BLOCK) {
-        int32_t first = -1;
-        const int64_t i = begin + Q.slot[q];
-        const int64_t b = V.row_ptr[i], e = V.row_ptr[i + 1];
-        const bool inf = scan_lane(V, sT, b, e, Q.m[q], first, w);
-        Q.hit[Q.slot[q]] = inf ? 1 : 0;
-        if (inf && V.history) ss_hist_append(V.history + i * SS_HIST_W, first);
+    if (nq > 0) {                                    // block-uniform
+        for (int q = tid; q < nq; q += SS_BLOCK) {
+            int32_t first = -1;
+            const uint32_t sl = Q.slot[q];
+            const bool hist_only = (sl & SS_QHIST) != 0u;
+            const int64_t i = begin + (sl & ~SS_QHIST);
+            const int64_t b = V.row_ptr[i], e = V.row_ptr[i + 1];
+            // with m = 0 the scan stops at the first infectious neighbour
+            const bool inf = scan_lane(V, C, b, e, Q.m[q], first, w);
+            if (!hist_only) Q.hit[sl] = inf ? 1 : 0;
+            if (inf && C->history) ss_hist_append(C->history + i * SS_HIST_W, first);
+            if (inf && !hist_only) dir_change(dir, V, V.s_code, V.s_target, e - b);
+        }
+        __syncthreads();
     }
-    __syncthreads();
+    uint32_t pmask = 0u;                             // bit a: agent a is infectious next tick
+    bool busy = false;                               // the tile must run next tick
 #pragma unroll
     for (int a = 0; a < A; ++a) {
         const int64_t ga = g0 + a * 64;
@@ -234,8 +408,30 @@ __device__ __forceinline__ void light_tile(const TickView V, const uint32_t *sT,
         const int64_t i = ga + lane;
         if (((scanned >> a) & 1u) && Q.hit[i - begin]) ns[a] = V.s_target;
         if (i < end) V.nxt[i] = ns[a];
-        const uint64_t word = __ballot(i < end && ns[a] == V.inf_code);
+        const bool inf = i < end && ns[a] == V.inf_code;
+        const uint64_t word = __ballot(inf);
         if (lane == 0) V.nb[ga >> 6] = word;
+        pmask |= (inf && i < V.n_local ? 1u : 0u) << a;
+        busy |= i < end && i < V.n_local && (ns[a] != s[a] || ns[a] == V.inf_code || trans_thr(C, ns[a]) != 0u);
+    }
+    // next tick's tile tag: a state changed (the other buffer is stale), an
+    // agent will draw or is infectious (its row is pushed every tick);
+    // candidates get theirs from the rows pushed below
+    if (V.push_next && __ballot(busy) && lane == 0) ((SS_G uint8_t *)C->tnext)[tix] = C->ptag;
+    if (V.push_next && __ballot(pmask != 0u)) {
+#pragma unroll 1
+        for (int a = 0; a < A; ++a) {
+            const uint64_t rows = __ballot((pmask >> a) & 1u);
+            if (!rows) continue;
+            const int64_t i = g0 + a * 64 + lane;
+            int64_t b = 0, e = 0;
+            if ((pmask >> a) & 1u) {
+                b = V.row_ptr[i];
+                e = V.row_ptr[i + 1];
+            }
+            const uint32_t n = push_rows(V, C, rows, b, e, lane);
+            if (lane == 0) w.pushed += n;
+        }
     }
 }
 
@@ -250,8 +446,8 @@ __device__ __forceinline__ void put_bits(SS_G uint64_t *nb, int64_t word, uint64
 // decide for their agent in parallel; the rows that need a scan are then
 // scanned one after another by the whole wavefront.
 #define SS_HPW 8
-__device__ __forceinline__ void heavy_tile(const TickView V, const uint32_t *sT, int64_t begin, int64_t end,
-                                           Work &w) {
+__device__ __forceinline__ void heavy_tile(const TickView V, const LCold *C, int64_t begin, int64_t end,
+                                           int64_t tix, Work &w, Dir &dir) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t i = begin + (int64_t)wv * SS_HPW + lane;
     const bool mine = lane < SS_HPW && i < end;
@@ -263,18 +459,31 @@ __device__ __forceinline__ void heavy_tile(const TickView V, const uint32_t *sT,
         s = V.cur[i];
         ns = s;
         if (i < V.n_local) {
+            b = V.row_ptr[i];
+            e = V.row_ptr[i + 1];
+            uint32_t kc = 1u;                        // pull: every susceptible agent is a candidate
+            if (V.cnt) {
+                kc = V.cnt[i];
+                if (kc != 0u) V.cnt[i] = 0u;
+            }
             if (s == V.s_code) {
                 ++w.s_agents;
-                b = V.row_ptr[i];
-                e = V.row_ptr[i + 1];
-                if (e > b) {
+                if (e > b && kc != 0u) {
                     ++w.draws;
                     m = draw(V, i);
-                    need = m < thr(V, sT, e - b);
-                    w.scans += need;
+                    if (V.cnt) {
+                        if (m < thr(V, C, (int64_t)kc)) {
+                            ns = V.s_target;
+                            need = C->history != nullptr;     // scan only for the first infectious neighbour
+                            m = 0u;
+                        }
+                    } else {
+                        need = m < thr(V, C, e - b);
+                        w.scans += need;
+                    }
                 }
             } else {
-                ns = transition(V, i, s, w);
+                ns = transition(V, C, i, s, w);
             }
         }
     }
@@ -286,13 +495,21 @@ __device__ __forceinline__ void heavy_tile(const TickView V, const uint32_t *sT,
         const int64_t el = ((int64_t)__shfl((int32_t)(e >> 32), l) << 32) | (uint32_t)__shfl((int32_t)e, l);
         const uint32_t ml = (uint32_t)__shfl((int32_t)m, l);
         int32_t first = -1;
-        const bool inf = scan_wave(V, sT, bl, el, ml, lane, first, w);
+        const bool inf = scan_wave(V, C, bl, el, ml, lane, first, w);
         if (lane == l && inf) {
             ns = V.s_target;
-            if (V.history) ss_hist_append(V.history + i * SS_HIST_W, first);
+            if (C->history) ss_hist_append(C->history + i * SS_HIST_W, first);
         }
     }
     if (mine) V.nxt[i] = ns;
+    if (mine && i < V.n_local) dir_change(dir, V, s, ns, e - b);
+    if (V.push_next) {
+        if (__ballot(mine && i < V.n_local && (ns != s || ns == V.inf_code || trans_thr(C, ns) != 0u)) && lane == 0)
+            ((SS_G uint8_t *)C->tnext)[tix] = C->ptag;
+        // rows of the wavefront's infectious agents, one after another, lanes striding
+        const uint32_t n = push_rows(V, C, __ballot(mine && i < V.n_local && ns == V.inf_code), b, e, lane);
+        if (lane == 0) w.pushed += n;
+    }
     const int64_t w0 = begin + (int64_t)wv * SS_HPW;
     if (w0 < end) {
         // the SS_HPW agents of a wavefront sit in one bitmap word
@@ -303,20 +520,41 @@ __device__ __forceinline__ void heavy_tile(const TickView V, const uint32_t *sT,
 
 // HUB tile: one agent, the whole block.  All 256 lanes count infectious
 // neighbours over rounds of 1024 entries; the round's total decides.
-__device__ __forceinline__ void hub_tile(const TickView V, const uint32_t *sT, int64_t i, Work &w, int *s_cnt,
-                                         long long *s_pos) {
+__device__ __forceinline__ void hub_tile(const TickView V, const LCold *C, int64_t i, int64_t tix, Work &w,
+                                         Dir &dir, int *s_cnt, long long *s_pos) {
+    // (s_cnt also carries the final state to the whole block for the push below)
     const int tid = threadIdx.x, lane = tid & 63;
     const int8_t s = V.cur[i];
     int8_t ns = s;
     int32_t first = -1;
+    int64_t b = 0, e = 0;
+    uint32_t kc = 1u;                                // pull: every susceptible agent is a candidate
+    if (V.cnt && i < V.n_local) {                    // block-uniform
+        // one thread reads and clears the count; the block agrees on it
+        if (tid == 0) {
+            const uint32_t c = V.cnt[i];
+            if (c != 0u) V.cnt[i] = 0u;
+            *s_cnt = (int)c;
+        }
+        __syncthreads();
+        kc = (uint32_t)*s_cnt;
+        __syncthreads();
+    }
     if (i < V.n_local) {
+        b = V.row_ptr[i];
+        e = V.row_ptr[i + 1];
         if (s == V.s_code) {
-            const int64_t b = V.row_ptr[i], e = V.row_ptr[i + 1];
             if (tid == 0) ++w.s_agents;
-            if (e > b) {
+            if (e > b && kc != 0u) {
                 if (tid == 0) ++w.draws;
-                const uint32_t m = draw(V, i);
-                if (m < thr(V, sT, e - b)) {
+                const uint32_t m0 = draw(V, i);
+                // push tick: the count decides; the scan below (m = 0) only
+                // looks for the first infectious neighbour of the history
+                const bool by_count = V.cnt != nullptr;
+                const bool hit_now = by_count && m0 < thr(V, C, (int64_t)kc);
+                if (hit_now) ns = V.s_target;
+                const uint32_t m = by_count ? 0u : m0;
+                if (by_count ? (hit_now && C->history) : m < thr(V, C, e - b)) {
                     if (tid == 0) ++w.scans;
                     int32_t k = 0;
                     for (int64_t base = b; base < e; base += 4 * SS_BLOCK) {
@@ -361,7 +599,7 @@ __device__ __forceinline__ void hub_tile(const TickView V, const uint32_t *sT, i
                         if (got > 0) {
                             if (first < 0) first = V.col_idx[p];
                             k += got;
-                            if (m < thr(V, sT, k)) {
+                            if (m < thr(V, C, k)) {
                                 ns = V.s_target;
                                 break;
                             }
@@ -370,47 +608,103 @@ __device__ __forceinline__ void hub_tile(const TickView V, const uint32_t *sT, i
                 }
             }
         } else if (tid == 0) {
-            ns = transition(V, i, s, w);
+            ns = transition(V, C, i, s, w);
         }
     }
     if (tid == 0) {
         V.nxt[i] = ns;
-        if (ns == V.s_target && s == V.s_code && i < V.n_local && V.history)
-            ss_hist_append(V.history + i * SS_HIST_W, first);
+        if (ns == V.s_target && s == V.s_code && i < V.n_local && C->history)
+            ss_hist_append(C->history + i * SS_HIST_W, first);
+        if (i < V.n_local) {
+            dir_change(dir, V, s, ns, e - b);
+            if (V.push_next && (ns != s || ns == V.inf_code || trans_thr(C, ns) != 0u))
+                ((SS_G uint8_t *)C->tnext)[tix] = C->ptag;
+        }
         const uint64_t bit = 1ull << (i & 63);
         put_bits(V.nb, i >> 6, ns == V.inf_code ? bit : 0ull, ns == V.inf_code ? 0ull : bit);
+    }
+    if (V.push_next) {                               // block-uniform
+        __syncthreads();
+        if (tid == 0) *s_cnt = ns == V.inf_code && i < V.n_local ? 1 : 0;
+        __syncthreads();
+        if (*s_cnt) {
+            Pusher(C).range(V.col_idx, b, e, tid, SS_BLOCK);
+            if (tid == 0) w.pushed += (uint32_t)(e - b);
+        }
+        __syncthreads();
     }
 }
 
 __device__ __forceinline__ void flush_work(unsigned long long *counters, const Work &w) {
-    uint32_t a = w.s_agents, d = w.draws, sc = w.scans, ed = w.edges;
+    uint32_t a = w.s_agents, d = w.draws, sc = w.scans, ed = w.edges, pu = w.pushed, sw = w.swept;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_down(a, off);
         d += __shfl_down(d, off);
         sc += __shfl_down(sc, off);
         ed += __shfl_down(ed, off);
+        pu += __shfl_down(pu, off);
+        sw += __shfl_down(sw, off);
     }
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(&counters[0], (unsigned long long)a);
         atomicAdd(&counters[1], (unsigned long long)d);
         atomicAdd(&counters[2], (unsigned long long)sc);
         atomicAdd(&counters[3], (unsigned long long)ed);
+        atomicAdd(&counters[4], (unsigned long long)pu);
+        atomicAdd(&counters[6], (unsigned long long)sw);
     }
+}
+
+// Block-wide sum of the direction statistics into one slot of the next
+// tick's accumulator (called by every thread of the block).
+__device__ __forceinline__ void flush_dir(Dir &d, unsigned long long *acc_next, unsigned long long *s_red) {
+    // signed changes, added in two's complement
+    unsigned long long a = (unsigned long long)d.ideg, c = (unsigned long long)(int64_t)d.sus;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off);
+        c += __shfl_down(c, off);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_red[2 * wv] = a;
+        s_red[2 * wv + 1] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && acc_next) {
+        unsigned long long ta = 0, tc = 0;
+        for (int q = 0; q < SS_BLOCK / 64; ++q) {
+            ta += s_red[2 * q];
+            tc += s_red[2 * q + 1];
+        }
+        const int slot = (int)(blockIdx.x % SS_ACC_SLOTS);
+        SS_G unsigned long long *g = (SS_G unsigned long long *)acc_next;
+        if (ta) __hip_atomic_fetch_add(g + 2 * slot, ta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tc) __hip_atomic_fetch_add(g + 2 * slot + 1, tc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    d = Dir{};
 }
 
 // Launch step k of a tick loop: every live replica r runs tick reps[r].tick0 + k.
 // Blocks stride over the tile list (hub tiles first, then heavy, then light),
 // so every block gets a share of the expensive tiles; no queue, no atomics.
+// Four waves per SIMD are asked for: the few values that then live in scratch
+// are read once per tile, while a fifth of the latency hiding would be lost
+// at three waves.
 template <int A>
-__global__ __launch_bounds__(SS_BLOCK) void tick_kernel(const ss_rep_dev *__restrict__ reps,
+__global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void tick_kernel(const ss_rep_dev *__restrict__ reps,
                                                         const ss_tile *__restrict__ tiles, int n_tiles, int k,
                                                         unsigned long long *counters) {
-    __shared__ uint32_t sT[SS_TLDS];
+    __shared__ Cold cold;
+    LCold *const C = (LCold *)&cold;
     __shared__ int s_cnt;
     __shared__ long long s_pos;
+    __shared__ unsigned long long s_red[2 * SS_BLOCK / 64];
     __shared__ ScanQueue Q;
     Work w;
+    Dir dir;
     int cached = -1;
     bool live = false;
     TickView V;
@@ -422,14 +716,33 @@ __global__ __launch_bounds__(SS_BLOCK) void tick_kernel(const ss_rep_dev *__rest
         const int64_t begin = nt.begin, end = nt.end;
         if (tix + (int)gridDim.x < n_tiles) nt = tiles[tix + gridDim.x];
         if (rep != cached) {                         // block-uniform
+            if (cached >= 0) flush_dir(dir, C->acc_next, s_red);   // barriers inside: C is free after
             const ss_rep_dev *R = reps + rep;
             const int64_t t = R->tick0 + k;
             live = t < R->tick_end;
             if (live) {
                 const int32_t last = R->inf_T_last;
-                __syncthreads();                     // the previous replica's table is no longer read
+                __syncthreads();                     // the previous replica's fields are no longer read
+                const int64_t t1 = t + 1;
                 if (threadIdx.x < SS_TLDS)
-                    sT[threadIdx.x] = R->inf_T[(int32_t)threadIdx.x < last ? (int32_t)threadIdx.x : last];
+                    C->T[threadIdx.x] = R->inf_T[(int32_t)threadIdx.x < last ? (int32_t)threadIdx.x : last];
+                if (threadIdx.x == 0) {
+                    C->inf_T = R->inf_T;
+                    C->history = R->history;
+                    C->pcnt = R->cnt[t1 & 1];
+                    C->tnext = R->tact[t1 & 1];
+                    C->n_hub = R->n_hub;
+                    C->h64 = R->h64;
+                    C->hub0 = R->hub0;
+                    C->heavy0 = R->heavy0;
+                    C->light0 = R->light0;
+                    C->light_shift = 31 - __clz(R->light_cap);
+                    C->ptag = ss_front_tag(t1);
+                }
+                if (threadIdx.x < 4) {
+                    C->tT[threadIdx.x] = R->trans_T[threadIdx.x];
+                    C->tN[threadIdx.x] = R->trans_next[threadIdx.x];
+                }
                 __syncthreads();
                 const bool odd = (t & 1) != 0;
                 V.row_ptr = (const SS_G int64_t *)R->row_ptr;
@@ -440,13 +753,30 @@ __global__ __launch_bounds__(SS_BLOCK) void tick_kernel(const ss_rep_dev *__rest
                 V.nxt = (SS_G int8_t *)(odd ? R->state[0] : R->state[1]);
                 V.cb = (const SS_G uint64_t *)(odd ? R->ibits[1] : R->ibits[0]);
                 V.nb = (SS_G uint64_t *)(odd ? R->ibits[0] : R->ibits[1]);
-                V.history = R->history;
-                V.inf_T = (const SS_G uint32_t *)R->inf_T;
                 V.deg16 = (const SS_G uint16_t *)R->deg16;
+                // tags of this tick exist when the previous tick decided so;
+                // this tick writes the next tick's tags when the rule says so
+                const uint32_t dec = R->cnt[0] ? R->dec[t & 1] : 0u;
+                const bool produce = push_rule(R, t);
+                V.cnt = dec == 0u ? nullptr : (SS_G uint32_t *)R->cnt[t & 1];
+                V.push_next = produce ? 1 : 0;
+                V.tag = ss_front_tag(t);
+                const uint8_t *tcur = dec == 1u ? R->tact[t & 1] : nullptr;
+                if (tix == R->lead_tile && threadIdx.x == 0) {
+                    // one block per replica: the decision for the next tick, the
+                    // next tick's statistics started from this tick's totals
+                    // (every tile adds its changes), the slot of t + 2 cleared
+                    if (R->cnt[0]) R->dec[(t + 1) & 1] = produce ? 1u : 0u;
+                    unsigned long long d, c;
+                    dir_totals(R, t, d, c);
+                    unsigned long long *nx = acc_slot(R, t + 1);
+                    atomicAdd(&nx[0], d);
+                    atomicAdd(&nx[1], c);
+                    unsigned long long *z = acc_slot(R, t + 2);
+                    for (int q = 0; q < SS_ACC_SLOTS * 2; ++q) z[q] = 0ull;
+                    if (counters && dec != 0u) atomicAdd(&counters[5], 1ull);
+                }
                 V.last = last;
-                V.tT0 = R->trans_T[0]; V.tT1 = R->trans_T[1]; V.tT2 = R->trans_T[2]; V.tT3 = R->trans_T[3];
-                V.tN0 = R->trans_next[0]; V.tN1 = R->trans_next[1];
-                V.tN2 = R->trans_next[2]; V.tN3 = R->trans_next[3];
                 V.s_code = R->s_code;
                 V.inf_code = R->inf_code;
                 V.s_target = R->s_target;
@@ -455,18 +785,124 @@ __global__ __launch_bounds__(SS_BLOCK) void tick_kernel(const ss_rep_dev *__rest
                 V.t_lo = (uint32_t)t;
                 V.t_hi = (uint32_t)((uint64_t)t >> 32);
                 V.n_local = R->n_local;
+                if (threadIdx.x == 0) {
+                    C->acc_next = R->acc ? acc_slot(R, t + 1) : nullptr;
+                    C->tcur = tcur;
+                }
+                __syncthreads();
             }
             cached = rep;
         }
         if (!live) continue;                         // replica finished: block-uniform
+        // push tick with tile tags: a tile without a tag has no candidate, no
+        // agent that draws and the same states in both buffers (block-uniform)
+        if (C->tcur && ((const SS_G uint8_t *)C->tcur)[tix] != V.tag) continue;
+        if (threadIdx.x == 0) w.swept += (uint32_t)(end - begin);
         if (kind == SS_TILE_LIGHT)
-            light_tile<A>(V, sT, begin, end, w, Q);
+            light_tile<A>(V, C, begin, end, tix, w, dir, Q);
         else if (kind == SS_TILE_HEAVY)
-            heavy_tile(V, sT, begin, end, w);
+            heavy_tile(V, C, begin, end, tix, w, dir);
         else
-            hub_tile(V, sT, begin, w, &s_cnt, &s_pos);
+            hub_tile(V, C, begin, tix, w, dir, &s_cnt, &s_pos);
     }
+    if (cached >= 0) flush_dir(dir, C->acc_next, s_red);
     if (counters) flush_work(counters, w);
+}
+
+// Infectious-neighbour counts of the current tick of one replica (cleared
+// before), pushed along the rows of its infectious agents, when *dec is set.
+// Rows [0, heavy) (the rows of degree >= 64 of a degree-sorted replica) are
+// walked by a wavefront each, the others by a thread.
+__global__ void count_init_kernel(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                                  const uint64_t *__restrict__ bits, int64_t n_local, int64_t heavy,
+                                  uint32_t *cnt, const uint32_t *dec) {
+    if (*dec == 0u) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = gt >> 6; i < heavy; i += nt >> 6) {
+        if (!((bits[i >> 6] >> (i & 63)) & 1ull)) continue;
+        const int64_t b = row_ptr[i], e = row_ptr[i + 1];
+        for (int64_t j = b + lane; j < e; j += 64) atomicAdd(&cnt[col[j]], 1u);
+    }
+    for (int64_t i = heavy + gt; i < n_local; i += nt) {
+        if (!((bits[i >> 6] >> (i & 63)) & 1ull)) continue;
+        const int64_t b = row_ptr[i], e = row_ptr[i + 1];
+        for (int64_t j = b; j < e; ++j) atomicAdd(&cnt[col[j]], 1u);
+    }
+}
+
+// The push decision of the current tick t from the statistics of tick t (one
+// thread): push when 16 * sum_deg(I) <= push_num * #S.
+__global__ void dec_init_kernel(const unsigned long long *acc_t, uint32_t push_num, uint32_t *dec) {
+    unsigned long long d = 0, c = 0;
+    for (int q = 0; q < SS_ACC_SLOTS; ++q) {
+        d += acc_t[2 * q];
+        c += acc_t[2 * q + 1];
+    }
+    // the counts of this tick come from here, not from a previous tick's tile tags
+    *dec = push_num != 0u && d * 16ull <= (unsigned long long)push_num * c ? 2u : 0u;
+}
+
+__global__ void dec_demote_kernel(uint32_t *dec) {
+    if (threadIdx.x < 2 && dec[threadIdx.x] == 1u) dec[threadIdx.x] = 2u;
+}
+
+// Direction statistics of one state buffer (local agents): acc[0] += sum of
+// degrees of agents in state inf, acc[1] += number of agents in state s_code.
+__global__ void acc_init_kernel(const int8_t *__restrict__ st, const int64_t *__restrict__ row_ptr, int64_t n_local,
+                                int8_t s_code, int8_t inf, unsigned long long *acc) {
+    unsigned long long d = 0, c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_local;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int8_t s = st[i];
+        if (s == inf) d += (unsigned long long)(row_ptr[i + 1] - row_ptr[i]);
+        c += s == s_code;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        d += __shfl_down(d, off);
+        c += __shfl_down(c, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (d) atomicAdd(&acc[0], d);
+        if (c) atomicAdd(&acc[1], c);
+    }
+}
+
+// 64-bit finaliser of splitmix64 (Steele, Lea, Flood 2014).
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Symmetry of a CSR as a multiset test: the sums of two independent hashes
+// over all entries (i, j) equal those over the mirrored entries (j, i) when
+// every entry has its mirror.  One thread per row.
+__global__ void symcheck_kernel(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col, int64_t n,
+                                unsigned long long *out) {
+    unsigned long long a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = row_ptr[i], e = row_ptr[i + 1];
+#pragma unroll 4
+        for (int64_t j = b; j < e; ++j) {
+            const uint64_t c = (uint32_t)col[j];
+            const uint64_t fwd = ((uint64_t)i << 32) | c, bwd = (c << 32) | (uint64_t)i;
+            a0 += mix64(fwd ^ 0x9E3779B97F4A7C15ull);
+            a1 += mix64(fwd * 0xD6E8FEB86659FD93ull + 0x2545F4914F6CDD1Dull);
+            b0 += mix64(bwd ^ 0x9E3779B97F4A7C15ull);
+            b1 += mix64(bwd * 0xD6E8FEB86659FD93ull + 0x2545F4914F6CDD1Dull);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_down(a0, off); a1 += __shfl_down(a1, off);
+        b0 += __shfl_down(b0, off); b1 += __shfl_down(b1, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&out[0], a0); atomicAdd(&out[1], a1);
+        atomicAdd(&out[2], b0); atomicAdd(&out[3], b1);
+    }
 }
 
 // Rebuild bitmap words [w0, w1) of one state buffer.
@@ -667,6 +1103,38 @@ hipError_t ss_launch_tick(const ss_rep_dev *reps, const ss_tile *tiles, int n_ti
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t ss_launch_count_init(const int64_t *row_ptr, const int32_t *col_idx, const uint64_t *bits,
+                                int64_t n_local, int64_t heavy, uint32_t *cnt, uint32_t *dec,
+                                const unsigned long long *acc_t, uint32_t push_num, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(dec_init_kernel, dim3(1), dim3(1), 0, s, acc_t, push_num, dec);
+    if (n_local <= 0) return hipGetLastError();
+    hipLaunchKernelGGL(count_init_kernel, dim3(grid), dim3(256), 0, s, row_ptr, col_idx, bits, n_local, heavy, cnt,
+                       (const uint32_t *)dec);
+    return hipGetLastError();
+}
+
+hipError_t ss_launch_dec_demote(uint32_t *dec, hipStream_t s) {
+    hipLaunchKernelGGL(dec_demote_kernel, dim3(1), dim3(64), 0, s, dec);
+    return hipGetLastError();
+}
+
+hipError_t ss_launch_acc_init(const int8_t *st, const int64_t *row_ptr, int64_t n_local, int8_t s_code, int8_t inf_code,
+                              unsigned long long *acc, int slot, int grid, hipStream_t s) {
+    if (n_local <= 0) return hipSuccess;
+    int g = (int)std::min<int64_t>(grid, (int64_t)grid_for(n_local, 256));
+    hipLaunchKernelGGL(acc_init_kernel, dim3(g), dim3(256), 0, s, st, row_ptr, n_local, s_code, inf_code,
+                       acc + slot * (SS_ACC_SLOTS * 2));
+    return hipGetLastError();
+}
+
+hipError_t ss_launch_symcheck(const int64_t *row_ptr, const int32_t *col_idx, int64_t n_rows,
+                              unsigned long long *out, int grid, hipStream_t s) {
+    if (n_rows <= 0) return hipSuccess;
+    int g = (int)std::min<int64_t>(grid, (int64_t)grid_for(n_rows, 256));
+    hipLaunchKernelGGL(symcheck_kernel, dim3(g), dim3(256), 0, s, row_ptr, col_idx, n_rows, out);
     return hipGetLastError();
 }
 

@@ -151,6 +151,19 @@ class Engine:
         self._ck(self._L.ss_set_tiling(self._h, hub_degree, int(heavy_degree), int(agents_per_lane),
                                        int(grid_blocks), 1 if use_graph else 0))
 
+    def set_push(self, ratio=0.25):
+        """Push-direction rule (default 0.25): a tick of a symmetric, ghost-free
+        replica writes the next tick's infectious-neighbour counts along the
+        rows of its infectious agents when sum(degree of infectious) <= ratio *
+        #susceptible; the next tick then decides from the counts and skips idle
+        tiles.  0 disables it.  Results are the same either way."""
+        self._ck(self._L.ss_set_push(self._h, float(ratio)))
+
+    def push_ok(self, rep):
+        out = ctypes.c_int32()
+        self._ck(self._L.ss_replica_push_ok(self._h, rep, ctypes.byref(out)))
+        return bool(out.value)
+
     def launch_shape(self):
         nt, g = ctypes.c_int32(), ctypes.c_int32()
         self._ck(self._L.ss_launch_shape(self._h, ctypes.byref(nt), ctypes.byref(g)))
@@ -167,8 +180,10 @@ class Engine:
         self._ck(self._L.ss_set_counting(self._h, 1 if on else 0))
 
     def read_counters(self):
-        """(susceptible agents visited, draws, scans started, CSR entries read); clears them."""
-        out = np.zeros(4, np.int64)
+        """(susceptible agents visited, draws, scans started, CSR entries read,
+        counts pushed, replica-ticks in the push direction, agents of the tiles
+        that ran); clears them."""
+        out = np.zeros(7, np.int64)
         self._ck(self._L.ss_read_counters(self._h, ptr(out)))
         return out
 

@@ -1,0 +1,169 @@
+"""The push direction (candidate tags from the infectious agents' rows) gives
+the same states as the pull direction and as the oracle, tick by tick, for
+every direction rule: never push, the default rule, and push whenever the
+replica allows it."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sagesim_amd import Engine, Replica, graphs, seir_model, sir_model, workloads
+
+pytestmark = pytest.mark.gpu
+
+RULES = (0.0, 0.25, 1.0e6)         # pull only, default, push whenever allowed
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def engines():
+    es = [Engine(0) for _ in RULES]
+    for e, r in zip(es, RULES):
+        e.set_push(r)
+    yield es
+    for e in es:
+        e.close()
+
+
+def first_diff(a, b):
+    d = np.nonzero(a != b)[0]
+    return None if d.size == 0 else (int(d[0]), int(a[d[0]]), int(b[d[0]]), int(d.size))
+
+
+def test_ba_push_pull_every_tick(engines):
+    """BA(200k, 5) SIR, 150 ticks: the three rules agree after every tick and
+    with the oracle's trace; the default rule mixes both directions."""
+    w = workloads.ba_sir(n=200_000, m=5, ticks=150, seed=21)
+    say("ba push/pull", w.graph.n, w.graph.nnz)
+    reps = [Replica(e, w.graph, w.state, w.model, w.seed) for e in engines]
+    assert all(e.push_ok(r.rep) for e, r in zip(engines, reps))
+    _, trace = O.sir_run(O.sir_model(w.beta, w.gamma, w.seed), w.graph.row_ptr, w.graph.col_idx,
+                         np.arange(w.graph.n), w.state, w.ticks, trace=True)
+    for e in engines:
+        e.set_counting(True)
+        e.read_counters()
+    for t in range(w.ticks):
+        for e in engines:
+            e.simulate(1)
+        if t % 5 == 4 or t < 12:
+            got = [r.states() for r in reps]
+            for g in got:
+                assert first_diff(g, trace[t]) is None, (t, first_diff(g, trace[t]))
+    push_ticks = [int(e.read_counters()[5]) for e in engines]
+    for e in engines:
+        e.set_counting(False)
+    say("push ticks per rule", push_ticks)
+    assert push_ticks[0] == 0
+    assert 0 < push_ticks[1] < w.ticks
+    assert push_ticks[2] == w.ticks
+    assert (trace[-1] == 2).sum() > w.graph.n // 10
+    for r in reps:
+        r.remove()
+
+
+def test_push_with_history(engines):
+    """Last-infector history (first infectious neighbour in CSR order) is the
+    oracle's when candidates come from the push direction."""
+    g = graphs.erdos_renyi(20_000, 0.0006, 5)
+    rng = np.random.default_rng(2)
+    st = np.where(rng.random(g.n) < 0.01, 1, 0).astype(np.int8)
+    hist0 = np.full((g.n, 32), -1, np.int32)
+    ids = np.arange(g.n)
+    want_h = hist0.copy()
+    want, _ = O.sir_run(O.sir_model(0.08, 0.1, 9), g.row_ptr, g.col_idx, ids, st, 60, history=want_h)
+    say("history under push", g.n, g.nnz)
+    for e in engines:
+        rep = Replica(e, g, st, sir_model(0.08, 0.1), 9, history=hist0)
+        e.simulate(60)
+        assert first_diff(rep.states(), want) is None
+        assert np.array_equal(rep.history(), want_h), np.argwhere(rep.history() != want_h)[:5]
+        rep.remove()
+    assert (want_h[:, 0] >= 0).sum() > 100
+
+
+def test_asymmetric_graph_stays_pull(engines):
+    """A directed CSR has no mirror entries: the replica never pushes and still
+    matches the oracle."""
+    rng = np.random.default_rng(4)
+    n = 30_000
+    u = rng.integers(0, n, 200_000)
+    v = rng.integers(0, n, 200_000)
+    g = graphs.from_edges(n, u, v, directed=True)
+    st = np.where(rng.random(n) < 0.01, 1, 0).astype(np.int8)
+    want, _ = O.sir_run(O.sir_model(0.1, 0.1, 3), g.row_ptr, g.col_idx, np.arange(n), st, 50)
+    e = engines[2]
+    rep = Replica(e, g, st, sir_model(0.1, 0.1), 3)
+    assert not e.push_ok(rep.rep)
+    e.set_counting(True)
+    e.read_counters()
+    e.simulate(50)
+    assert int(e.read_counters()[5]) == 0
+    e.set_counting(False)
+    assert first_diff(rep.states(), want) is None
+    rep.remove()
+
+
+def test_ensemble_mixed_directions(engines):
+    """Replicas of different size, model, tick offset and symmetry in one
+    launch, always-push rule: each equals its own oracle run."""
+    e = engines[2]
+    cases = []
+    for k, (n, m) in enumerate([(5_000, 2), (40_000, 6), (120_000, 3)]):
+        w = workloads.ba_sir(n=n, m=m, beta=0.02 + 0.01 * k, gamma=0.1 + 0.03 * k, ticks=80, seed=30 + k)
+        cases.append((w.graph, w.state, w.model, O.sir_model(w.beta, w.gamma, w.seed), w.seed))
+    ws = workloads.ws_seir(n=50_000, ticks=80, seed=12)
+    cases.append((ws.graph, ws.state, ws.model, O.seir_model(ws.beta, ws.sigma, ws.gamma, ws.seed), ws.seed))
+    rng = np.random.default_rng(8)
+    n = 20_000
+    gd = graphs.from_edges(n, rng.integers(0, n, 120_000), rng.integers(0, n, 120_000), directed=True)
+    sd = np.where(rng.random(n) < 0.02, 1, 0).astype(np.int8)
+    cases.append((gd, sd, sir_model(0.1, 0.1), O.sir_model(0.1, 0.1, 5), 5))
+    reps = [Replica(e, g, st, mdl, seed) for g, st, mdl, _, seed in cases]
+    say("ensemble mixed", [g.n for g, *_ in cases])
+    e.simulate(80)
+    for (g, st, _, om, _), r in zip(cases, reps):
+        want, _ = O.sir_run(om, g.row_ptr, g.col_idx, np.arange(g.n), st, 80)
+        assert first_diff(r.states(), want) is None, (g.n, first_diff(r.states(), want))
+    assert [e.push_ok(r.rep) for r in reps] == [True] * 4 + [False]
+    for r in reps:
+        r.remove()
+
+
+def test_tile_rebuild_while_pushing(engines):
+    """Adding and removing a replica rebuilds the tile list between ticks of a
+    replica that is pushing with tile tags; every state stays the oracle's."""
+    e = engines[2]
+    wa = workloads.ba_sir(n=100_000, m=4, ticks=160, seed=41)
+    wb = workloads.ba_sir(n=30_000, m=3, beta=0.05, ticks=60, seed=42)
+    oa = O.sir_model(wa.beta, wa.gamma, wa.seed)
+    ob = O.sir_model(wb.beta, wb.gamma, wb.seed)
+    ra = Replica(e, wa.graph, wa.state, wa.model, wa.seed)
+    e.simulate(60)
+    rb = Replica(e, wb.graph, wb.state, wb.model, wb.seed)
+    say("tile rebuild", wa.graph.n, wb.graph.n)
+    e.simulate(60)
+    want_b, _ = O.sir_run(ob, wb.graph.row_ptr, wb.graph.col_idx, np.arange(wb.graph.n), wb.state, 60)
+    assert first_diff(rb.states(), want_b) is None
+    rb.remove()
+    e.simulate(40)
+    want_a, _ = O.sir_run(oa, wa.graph.row_ptr, wa.graph.col_idx, np.arange(wa.graph.n), wa.state, 160)
+    assert first_diff(ra.states(), want_a) is None
+    ra.remove()
+
+
+def test_si_model_keeps_pushing(engines):
+    """SI (gamma = 0): infectious agents never draw, yet their rows are pushed
+    every tick; states equal the oracle's under every rule."""
+    w = workloads.ba_sir(n=60_000, m=3, beta=0.02, gamma=0.0, ticks=90, seed=51)
+    om = O.sir_model(w.beta, 0.0, w.seed)
+    _, trace = O.sir_run(om, w.graph.row_ptr, w.graph.col_idx, np.arange(w.graph.n), w.state, w.ticks, trace=True)
+    say("si model", w.graph.n)
+    for e in engines:
+        rep = Replica(e, w.graph, w.state, sir_model(w.beta, 0.0), w.seed)
+        for t0 in range(0, w.ticks, 30):
+            e.simulate(30)
+            assert first_diff(rep.states(), trace[t0 + 29]) is None, (t0, first_diff(rep.states(), trace[t0 + 29]))
+        rep.remove()
+    assert 0 < (trace[-1] == 1).sum() < w.graph.n
