@@ -45,14 +45,10 @@ def serving_requests(rate=50.0, duration=600.0, lo=50_000, hi=5_000_000, seed=77
 
 
 def footprint(graph, with_history=False):
-    """Device bytes of a replica (mirrors ss_replica_footprint)."""
-    n = graph.n
-    nt = graph.n_cols
-    words = (nt + 63) // 64
-    b = 8 * (nt + 1) + 4 * graph.nnz + 6 * n + 3 * nt + 16 * words + 4 * (int(graph.degrees().max(initial=0)) + 1)
-    if with_history:
-        b += 2 * 4 * n * 32
-    return b
+    """Device bytes of a replica, as the library accounts them (ss_replica_footprint)."""
+    from ._lib import step_lib
+    return int(step_lib().ss_replica_footprint(graph.n, graph.n_cols - graph.n, graph.nnz, 1 if with_history else 0,
+                                               int(graph.degrees().max(initial=0))))
 
 
 @dataclass
