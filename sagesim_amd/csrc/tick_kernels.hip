@@ -274,15 +274,23 @@ __device__ __forceinline__ bool scan_wave(const TickView V, const LCold *C, int6
     return false;
 }
 
-// Neighbour scans of a light tile, gathered in LDS so that all threads of the
-// block scan concurrently instead of each lane walking its own agents in turn.
+// Neighbour scans of a light tile, gathered in LDS so that the lanes of a
+// wavefront scan concurrently instead of each lane walking its own agents in
+// turn.  Each wavefront owns the queue section of its own agents.
 #define SS_LIGHT_MAX (SS_BLOCK * 8)
 struct ScanQueue {
     uint32_t m[SS_LIGHT_MAX];      // the agent's draw
     uint16_t slot[SS_LIGHT_MAX];   // agent - tile begin
     uint8_t hit[SS_LIGHT_MAX];     // scan result by slot
-    int n;
 };
+
+// Orders a wavefront's LDS writes before its later LDS reads by other lanes
+// (LDS operations of one wavefront complete in order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // LIGHT tile: A agents per lane (agents g0 + a*64 + lane).  Round trip 1
 // reads the states (and, on a push tick, the infectious-neighbour counts,
@@ -299,8 +307,9 @@ __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int
                                            int64_t tix, Work &w, Dir &dir, ScanQueue &Q) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t g0 = begin + (int64_t)wv * 64 * A;
-    if (tid == 0) Q.n = 0;
-    __syncthreads();
+    const int sec = wv * 64 * A;                     // this wavefront's queue section
+    const uint64_t below = (1ull << lane) - 1ull;    // lanes before this one
+    int nq = 0;                                      // wave-uniform queue length
     int8_t s[A], ns[A];
     uint32_t cand = 0u;                              // bit a: agent a is a susceptible candidate
     uint32_t key[A], khi[A], kc[A];                  // kc: infectious neighbours (push tick)
@@ -346,30 +355,28 @@ __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int
     for (int a = 0; a < A; ++a) {
         const int64_t i = g0 + a * 64 + lane;
         ns[a] = s[a];
+        bool enq = false, hist = false;
+        uint32_t m = 0u;
         if (i < end && i < V.n_local) {
             if (s[a] == V.s_code) {
                 ++w.s_agents;
                 if (deg[a] > 0 && ((cand >> a) & 1u)) {
                     ++w.draws;
-                    const uint32_t m = draw_key(V, key[a], khi[a]);
+                    m = draw_key(V, key[a], khi[a]);
                     if (V.cnt) {
                         if (m < thr(V, C, (int64_t)kc[a])) {
                             ns[a] = V.s_target;
                             dir_change(dir, V, s[a], ns[a], deg[a] == 65535u ? (int64_t)(V.row_ptr[i + 1] - V.row_ptr[i])
                                                                              : (int64_t)deg[a]);
-                            if (C->history) {
-                                const int q = atomicAdd(&Q.n, 1);
-                                Q.m[q] = 0u;
-                                Q.slot[q] = (uint16_t)((uint32_t)(i - begin) | SS_QHIST);
-                            }
+                            // the history's first infectious neighbour (m = 0 stops the scan there)
+                            enq = hist = C->history != nullptr;
+                            m = 0u;
                         }
                     } else if (m < thr(V, C, deg[a] == 65535u ? (int64_t)V.last : (int64_t)deg[a])) {
                         // a capped degree (65535) bounds with the last table entry
+                        enq = true;
                         scanned |= 1u << a;
                         ++w.scans;
-                        const int q = atomicAdd(&Q.n, 1);
-                        Q.m[q] = m;
-                        Q.slot[q] = (uint16_t)(i - begin);
                     }
                 }
             } else {
@@ -381,23 +388,29 @@ __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int
                 dir_change(dir, V, s[a], ns[a], deg[a]);
             }
         }
+        const uint64_t qm = __ballot(enq);
+        if (enq) {
+            const int q = sec + nq + __popcll(qm & below);
+            Q.m[q] = m;
+            Q.slot[q] = (uint16_t)((uint32_t)(i - begin) | (hist ? SS_QHIST : 0u));
+        }
+        nq += __popcll(qm);
     }
-    __syncthreads();
-    const int nq = Q.n;
-    if (nq > 0) {                                    // block-uniform
-        for (int q = tid; q < nq; q += SS_BLOCK) {
+    if (nq > 0) {                                    // wave-uniform
+        wave_sync();
+        for (int q = lane; q < nq; q += 64) {
             int32_t first = -1;
-            const uint32_t sl = Q.slot[q];
+            const uint32_t sl = Q.slot[sec + q];
             const bool hist_only = (sl & SS_QHIST) != 0u;
             const int64_t i = begin + (sl & ~SS_QHIST);
             const int64_t b = V.row_ptr[i], e = V.row_ptr[i + 1];
             // with m = 0 the scan stops at the first infectious neighbour
-            const bool inf = scan_lane(V, C, b, e, Q.m[q], first, w);
+            const bool inf = scan_lane(V, C, b, e, Q.m[sec + q], first, w);
             if (!hist_only) Q.hit[sl] = inf ? 1 : 0;
             if (inf && C->history) ss_hist_append(C->history + i * SS_HIST_W, first);
             if (inf && !hist_only) dir_change(dir, V, V.s_code, V.s_target, e - b);
         }
-        __syncthreads();
+        wave_sync();
     }
     uint32_t pmask = 0u;                             // bit a: agent a is infectious next tick
     bool busy = false;                               // the tile must run next tick
