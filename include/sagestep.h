@@ -119,7 +119,7 @@ int ss_simulate(ss_ctx *ctx, int64_t ticks);
 /* Launch shape.  Rows are expected in descending-degree order.  The rows of
  * degree >= hub_degree at the front get a whole block each; the following
  * rows of degree >= heavy_degree get a wavefront each for their neighbour
- * scan; the rest run agents_per_lane (1, 2, 4 or 8) agents per lane.  Degrees
+ * scan; the rest run agents_per_lane (1, 2, 4, 8 or 16) agents per lane.  Degrees
  * are one of 8,16,24,32,48,64,96,128,192,256,512,1024,2048,4096,16384,2^30.
  * grid_blocks = 0 sizes the persistent grid from occupancy; use_graph = 0
  * launches tick by tick instead of replaying a captured hipGraph. */

@@ -29,6 +29,9 @@ static thread_local std::string g_err;
 
 namespace {
 
+// Length of a per-agent array padded to whole groups of 64 agents.
+inline int64_t pad_agents(int64_t n) { return (n + 63) / 64 * 64; }
+
 template <class T>
 hipError_t dalloc(ss_ctx *ctx, ss_replica *r, T **p, size_t count) {
     *p = nullptr;
@@ -146,7 +149,7 @@ int init_acc(ss_ctx *ctx, ss_replica &r) {
         const int par = (int)(r.tick & 1);
         const uint32_t num = r.dev.push_ok ? ctx->push_num : 0u;
         for (int b = 0; b < 2; ++b)
-            SS_HIP(ctx, hipMemsetAsync(r.cnt[b], 0, sizeof(uint32_t) * (size_t)r.n_local, ctx->stream));
+            SS_HIP(ctx, hipMemsetAsync(r.cnt[b], 0, sizeof(uint32_t) * (size_t)pad_agents(r.n_local), ctx->stream));
         SS_HIP(ctx, ss_launch_count_init(r.row_ptr, r.col_idx, r.ibits[par], r.n_local, heavy_prefix(r, 64),
                                          r.cnt[par], r.dec + par, r.acc + slot * (SS_ACC_SLOTS * 2), num,
                                          ctx->num_cus * 8, ctx->stream));
@@ -405,8 +408,9 @@ int ss_device_info(ss_ctx *ctx, int32_t *num_cus, int64_t *total_mem, int64_t *c
 int64_t ss_replica_footprint(int64_t n_local, int64_t n_ghost, int64_t nnz, int32_t with_history, int64_t max_deg) {
     int64_t n_total = n_local + n_ghost;
     int64_t words = (n_total + 63) / 64;
-    int64_t b = 8 * (n_total + 1) + 4 * nnz + 6 * n_local + 3 * n_total + 2 * 8 * words +
-                4 * (std::min<int64_t>(max_deg, 1 << 24) + 1) + 8 * SS_ACC_WORDS + (n_ghost == 0 ? 8 * n_local + 8 : 0);
+    int64_t b = 8 * (n_total + 1) + 4 * nnz + 6 * pad_agents(n_local) + 2 * pad_agents(n_total) + n_total +
+                2 * 8 * words + 4 * (std::min<int64_t>(max_deg, 1 << 24) + 1) + 8 * SS_ACC_WORDS +
+                (n_ghost == 0 ? 8 * pad_agents(n_local) + 8 : 0);
     if (with_history) b += 2 * 4 * n_local * SS_HISTORY_WIDTH;
     return b;
 }
@@ -464,18 +468,19 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
     } while (0)
     R_HIP(dalloc(ctx, &r, &r.row_ptr, (size_t)nt + 1));
     R_HIP(dalloc(ctx, &r, &r.col_idx, (size_t)nnz));
-    R_HIP(dalloc(ctx, &r, &r.rng_lo, (size_t)nl));
-    R_HIP(dalloc(ctx, &r, &r.deg16, (size_t)nl));
+    // per-agent arrays the light tiles read as 16-agent vectors are padded
+    R_HIP(dalloc(ctx, &r, &r.rng_lo, (size_t)pad_agents(nl)));
+    R_HIP(dalloc(ctx, &r, &r.deg16, (size_t)pad_agents(nl)));
     if (need_hi) R_HIP(dalloc(ctx, &r, &r.rng_hi, (size_t)nl));
     for (int b = 0; b < 2; ++b) {
-        R_HIP(dalloc(ctx, &r, &r.state[b], (size_t)nt));
+        R_HIP(dalloc(ctx, &r, &r.state[b], (size_t)pad_agents(nt)));
         R_HIP(dalloc(ctx, &r, &r.ibits[b], (size_t)r.n_words));
     }
     R_HIP(dalloc(ctx, &r, &r.init_state, (size_t)nt));
     R_HIP(dalloc(ctx, &r, &r.acc, (size_t)SS_ACC_WORDS));
     if (ng == 0 && nl > 0) {
-        R_HIP(dalloc(ctx, &r, &r.cnt[0], (size_t)nl));
-        R_HIP(dalloc(ctx, &r, &r.cnt[1], (size_t)nl));
+        R_HIP(dalloc(ctx, &r, &r.cnt[0], (size_t)pad_agents(nl)));
+        R_HIP(dalloc(ctx, &r, &r.cnt[1], (size_t)pad_agents(nl)));
         R_HIP(dalloc(ctx, &r, &r.dec, (size_t)2));
         R_HIP(hipMemset(r.dec, 0, 2 * sizeof(uint32_t)));
     }
@@ -684,8 +689,8 @@ int ss_history_download(ss_ctx *ctx, int32_t rep, int32_t *out, int64_t n_rows) 
 int ss_set_tiling(ss_ctx *ctx, int32_t hub_degree, int32_t heavy_degree, int32_t apl, int32_t grid_blocks,
                   int32_t use_graph) {
     if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
-    if (apl != 1 && apl != 2 && apl != 4 && apl != 8)
-        SS_FAIL(ctx, SS_ERR_INVALID, "agents_per_lane must be 1, 2, 4 or 8");
+    if (apl != 1 && apl != 2 && apl != 4 && apl != 8 && apl != 16)
+        SS_FAIL(ctx, SS_ERR_INVALID, "agents_per_lane must be 1, 2, 4, 8 or 16");
     bool hub_ok = false, heavy_ok = false;
     for (int32_t v : SS_THRESHOLDS) {
         hub_ok |= v == hub_degree;

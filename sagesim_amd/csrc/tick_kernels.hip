@@ -448,6 +448,300 @@ __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int
     }
 }
 
+// Wide LIGHT tile: each wavefront owns a chunk of 1024 consecutive agents and
+// lane l the 16 agents [16 l, 16 l + 16) of it, so states, counts, degrees and
+// RNG keys arrive as 16-byte vector loads, one round trip for the chunk.  The
+// agents that draw (susceptible candidates, agents with a spontaneous
+// transition) are packed into the wavefront's LDS queue, so the draws run on
+// full wavefronts whatever the mix of states; the rows that must be scanned
+// are packed again (in place) and scanned one per lane, long rows by the
+// whole wavefront.  Results return through a byte per chunk slot.
+#define SS_WIDE_A 16
+#define SS_WIDE_CHUNK (64 * SS_WIDE_A)
+#define SS_WIDE_LONG 64          // rows longer than this are scanned by the whole wavefront
+struct alignas(16) WideQueue {
+    // draw entry: x = RNG key (pull) or infectious-neighbour count (push);
+    //   y = slot | transition << 10 | (state & 3) << 11 | deg16 << 16 (deg16 only on pull)
+    // scan entry: x = draw; y = slot | history-only << 10 | deg16 << 16
+    uint2 e[SS_BLOCK / 64][SS_WIDE_CHUNK];
+    uint8_t res[SS_BLOCK / 64][SS_WIDE_CHUNK];   // next state by chunk slot (queued agents only)
+};
+
+// 0xFF in byte b of the result for every set bit b of n (n < 16).
+__device__ __forceinline__ uint32_t byte_mask4(uint32_t n) {
+    return ((n * 0x00204081u) & 0x01010101u) * 0xFFu;
+}
+
+// Bit b of the result set where byte b of x equals the byte whose copies fill y.
+__device__ __forceinline__ uint32_t eq_bytes4(uint32_t x, uint32_t y) {
+    const uint32_t t = x ^ y;
+    const uint32_t z = ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t) & 0x80808080u;   // 0x80 where a byte of t is 0
+    return (((z >> 7) * 0x00204081u) >> 21) & 0xFu;
+}
+
+// 16-byte global loads and stores (p 16-byte aligned)
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 ld4(const SS_G void *p) {
+    const v4u32 x = *(const SS_G v4u32 *)p;
+    return make_uint4(x.x, x.y, x.z, x.w);
+}
+__device__ __forceinline__ void st4(SS_G void *p, const uint4 v) {
+    v4u32 x;
+    x.x = v.x; x.y = v.y; x.z = v.z; x.w = v.w;
+    *(SS_G v4u32 *)p = x;
+}
+
+__device__ __forceinline__ uint32_t u4_word(const uint4 v, int q) {
+    return q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w;
+}
+
+// Degree of local agent i from its 16-bit copy (65535: read the row bounds).
+__device__ __forceinline__ int64_t deg_of(const TickView &V, int64_t i, uint32_t d16) {
+    return d16 == 65535u ? V.row_ptr[i + 1] - V.row_ptr[i] : (int64_t)d16;
+}
+
+__device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int64_t begin, int64_t end, int64_t tix,
+                                           Work &w, Dir &dir, WideQueue &Q) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t c0 = begin + (int64_t)wv * SS_WIDE_CHUNK;      // chunk start, a multiple of 64
+    if (c0 >= end) return;                                        // wave-uniform: no block barrier below
+    uint2 *const E = Q.e[wv];
+    uint8_t *const res = Q.res[wv];
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int64_t i0 = c0 + 16 * lane;
+    const int nv = (int)(end - i0 < 0 ? 0 : end - i0 > 16 ? 16 : end - i0);           // agents of the lane
+    const int nl = (int)(V.n_local - i0 < 0 ? 0 : V.n_local - i0 > nv ? nv : V.n_local - i0);   // local rows
+    const uint32_t vmask = (1u << nv) - 1u, lmask = (1u << nl) - 1u;
+    const bool pull = V.cnt == nullptr;
+    // round trip 1: states; counts (push) or RNG keys and degrees (pull).  The
+    // arrays are padded to whole 16-agent groups.
+    uint4 sv = make_uint4(0u, 0u, 0u, 0u);
+    if (nv > 0) sv = ld4(V.cur + i0);
+    uint32_t a[16];
+    uint32_t dg[8];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) a[q] = 0u;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) dg[q] = 0u;
+    if (nl > 0) {
+        if (!pull) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint4 x = ld4(V.cnt + i0 + 4 * q);
+                a[4 * q] = x.x; a[4 * q + 1] = x.y; a[4 * q + 2] = x.z; a[4 * q + 3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint4 x = ld4(V.rng_lo + i0 + 4 * q);
+                a[4 * q] = x.x; a[4 * q + 1] = x.y; a[4 * q + 2] = x.z; a[4 * q + 3] = x.w;
+            }
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const uint4 x = ld4(V.deg16 + i0 + 8 * q);
+                dg[4 * q] = x.x; dg[4 * q + 1] = x.y; dg[4 * q + 2] = x.z; dg[4 * q + 3] = x.w;
+            }
+        }
+    }
+    if (!pull && nl > 0) {
+        // counts are cleared as they are read, ready for the tick after next
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if ((a[4 * q] | a[4 * q + 1] | a[4 * q + 2] | a[4 * q + 3]) != 0u)
+                st4(V.cnt + i0 + 4 * q, make_uint4(0u, 0u, 0u, 0u));
+    }
+    // states with a spontaneous transition, as a bit set over state codes 0..3
+    uint32_t tbits = 0u;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) tbits |= (C->tT[s] != 0u ? 1u : 0u) << s;
+    uint32_t smask = 0u, tmask = 0u, cmask = 0u;     // susceptible, transition, candidate
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const uint32_t s = (u4_word(sv, j >> 2) >> (8 * (j & 3))) & 0xFFu;
+        const bool is_s = (int8_t)s == V.s_code;
+        smask |= (is_s ? 1u : 0u) << j;
+        tmask |= (!is_s && s < 4u && ((tbits >> s) & 1u) ? 1u : 0u) << j;
+        const uint32_t nz = pull ? ((dg[j >> 1] >> (16 * (j & 1))) & 0xFFFFu) : a[j];
+        cmask |= (nz != 0u ? 1u : 0u) << j;
+    }
+    smask &= lmask;
+    tmask &= lmask;
+    const uint32_t enq = (smask & cmask) | tmask;
+    w.s_agents += (uint32_t)__popc(smask);
+    // the lane's queue position: exclusive prefix over lanes of popc(enq) (<= 16)
+    const uint32_t ne = (uint32_t)__popc(enq);
+    uint32_t pos = 0u, nq = 0u;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+        const uint64_t B = __ballot((ne >> b) & 1u);
+        pos += (uint32_t)__popcll(B & below) << b;
+        nq += (uint32_t)__popcll(B) << b;
+    }
+    if (enq) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if ((enq >> j) & 1u) {
+                const uint32_t s = (u4_word(sv, j >> 2) >> (8 * (j & 3))) & 3u;
+                const uint32_t d = pull ? ((dg[j >> 1] >> (16 * (j & 1))) & 0xFFFFu) : 0u;
+                E[pos] = make_uint2(a[j], (uint32_t)(16 * lane + j) | ((tmask >> j) & 1u) << 10 | s << 11 | d << 16);
+                ++pos;
+            }
+        }
+    }
+    bool busy = false;                               // the tile must run next tick
+    uint32_t nsq = 0u;                               // wave-uniform scan-queue length
+    if (nq > 0u) {                                   // wave-uniform
+        wave_sync();
+        // draws, a full wavefront of queued agents per round; the rows to scan
+        // are packed over the entries already read
+        for (uint32_t r0 = 0; r0 < nq; r0 += 64) {
+            const uint32_t q = r0 + (uint32_t)lane;
+            bool sc = false;
+            uint2 se = make_uint2(0u, 0u);
+            if (q < nq) {
+                const uint2 en = E[q];
+                const uint32_t slot = en.y & 1023u;
+                const bool trans = ((en.y >> 10) & 1u) != 0u;
+                const int64_t i = c0 + slot;
+                const uint32_t key = pull ? en.x : V.rng_lo[i];
+                const uint32_t khi = V.rng_hi ? V.rng_hi[i] : 0u;
+                ++w.draws;
+                const uint32_t m = draw_key(V, key, khi);
+                int8_t s, ns;
+                if (trans) {
+                    s = (int8_t)((en.y >> 11) & 3u);
+                    ns = m < C->tT[s] ? trans_next(C, s) : s;
+                } else {
+                    s = ns = V.s_code;
+                    if (!pull) {
+                        if (m < thr(V, C, (int64_t)en.x)) {
+                            ns = V.s_target;
+                            if (C->history) {
+                                // the history's first infectious neighbour (draw 0 stops the scan there)
+                                sc = true;
+                                se = make_uint2(0u, slot | 1u << 10 | (uint32_t)V.deg16[i] << 16);
+                            }
+                        }
+                    } else {
+                        const uint32_t d = en.y >> 16;
+                        // a capped degree (65535) bounds with the last table entry
+                        if (m < thr(V, C, d == 65535u ? (int64_t)V.last : (int64_t)d)) {
+                            sc = true;
+                            se = make_uint2(m, slot | d << 16);
+                            ++w.scans;
+                        }
+                    }
+                }
+                if (ns != s)
+                    dir_change(dir, V, s, ns, deg_of(V, i, pull ? en.y >> 16 : (uint32_t)V.deg16[i]));
+                busy |= ns != s || ns == V.inf_code || trans_thr(C, ns) != 0u;
+                res[slot] = (uint8_t)ns;
+            }
+            const uint64_t sm = __ballot(sc);
+            if (sc) E[nsq + (uint32_t)__popcll(sm & below)] = se;
+            nsq += (uint32_t)__popcll(sm);
+        }
+    }
+    if (nsq > 0u) {                                  // wave-uniform
+        wave_sync();
+        for (uint32_t r0 = 0; r0 < nsq; r0 += 64) {
+            const uint32_t q = r0 + (uint32_t)lane;
+            bool lng = false;
+            int64_t b = 0, e = 0;
+            uint32_t m = 0u, y = 0u;
+            if (q < nsq) {
+                const uint2 en = E[q];
+                m = en.x;
+                y = en.y;
+                const int64_t i = c0 + (y & 1023u);
+                const uint32_t d = y >> 16;
+                b = V.row_ptr[i];
+                e = d == 65535u ? V.row_ptr[i + 1] : b + (int64_t)d;
+                lng = e - b > SS_WIDE_LONG;
+            }
+            bool inf = false;
+            int32_t first = -1;
+            if (q < nsq && !lng) inf = scan_lane(V, C, b, e, m, first, w);
+            uint64_t lm = __ballot(lng);
+            while (lm) {
+                const int l = __ffsll((unsigned long long)lm) - 1;
+                lm &= lm - 1;
+                const int64_t bl = ((int64_t)__shfl((int32_t)(b >> 32), l) << 32) | (uint32_t)__shfl((int32_t)b, l);
+                const int64_t el = ((int64_t)__shfl((int32_t)(e >> 32), l) << 32) | (uint32_t)__shfl((int32_t)e, l);
+                const uint32_t ml = (uint32_t)__shfl((int32_t)m, l);
+                int32_t fl = -1;
+                const bool il = scan_wave(V, C, bl, el, ml, lane, fl, w);
+                if (lane == l) {
+                    inf = il;
+                    first = fl;
+                }
+            }
+            if (q < nsq && inf) {
+                const uint32_t slot = y & 1023u;
+                const int64_t i = c0 + slot;
+                if (((y >> 10) & 1u) == 0u) {
+                    res[slot] = (uint8_t)V.s_target;
+                    dir_change(dir, V, V.s_code, V.s_target, e - b);
+                    busy = true;
+                }
+                if (C->history) ss_hist_append(C->history + i * SS_HIST_W, first);
+            }
+        }
+    }
+    // next states: the queued agents' from LDS, the others unchanged
+    uint4 nsv = sv;
+    if (nq > 0u) {
+        wave_sync();
+        if (enq) {
+            const uint4 r = *(const uint4 *)(res + 16 * lane);
+            const uint32_t m0 = byte_mask4(enq & 15u), m1 = byte_mask4((enq >> 4) & 15u);
+            const uint32_t m2 = byte_mask4((enq >> 8) & 15u), m3 = byte_mask4(enq >> 12);
+            nsv.x = (r.x & m0) | (sv.x & ~m0);
+            nsv.y = (r.y & m1) | (sv.y & ~m1);
+            nsv.z = (r.z & m2) | (sv.z & ~m2);
+            nsv.w = (r.w & m3) | (sv.w & ~m3);
+        }
+    }
+    if (nv == 16) {
+        st4(V.nxt + i0, nsv);
+    } else if (nv > 0) {
+#pragma unroll 1
+        for (int j = 0; j < nv; ++j) V.nxt[i0 + j] = (int8_t)((u4_word(nsv, j >> 2) >> (8 * (j & 3))) & 0xFFu);
+    }
+    // infectious bits of the lane's agents; lanes 4g .. 4g+3 fill bitmap word g of the chunk
+    const uint32_t X = 0x01010101u * (uint32_t)(uint8_t)V.inf_code;
+    const uint32_t ib = (eq_bytes4(nsv.x, X) | eq_bytes4(nsv.y, X) << 4 | eq_bytes4(nsv.z, X) << 8 |
+                         eq_bytes4(nsv.w, X) << 12) & vmask;
+    {
+        uint32_t lo = (lane & 2) ? 0u : ib << (16 * (lane & 1));
+        uint32_t hi = (lane & 2) ? ib << (16 * (lane & 1)) : 0u;
+        lo |= (uint32_t)__shfl_xor((int32_t)lo, 1);
+        hi |= (uint32_t)__shfl_xor((int32_t)hi, 1);
+        lo |= (uint32_t)__shfl_xor((int32_t)lo, 2);
+        hi |= (uint32_t)__shfl_xor((int32_t)hi, 2);
+        if ((lane & 3) == 0 && nv > 0) V.nb[i0 >> 6] = (uint64_t)hi << 32 | lo;
+    }
+    if (V.push_next) {
+        uint32_t pend = ib & lmask;                  // infectious local agents: their rows are pushed
+        // next tick's tile tag: a state changed (the other buffer is stale), an
+        // agent will draw or is infectious (its row is pushed every tick);
+        // candidates get theirs from the rows pushed below
+        if (__ballot(busy || pend != 0u) && lane == 0) ((SS_G uint8_t *)C->tnext)[tix] = C->ptag;
+        while (__ballot(pend != 0u)) {
+            const bool has = pend != 0u;
+            int64_t b = 0, e = 0;
+            if (has) {
+                const int64_t i = i0 + (__ffs(pend) - 1);
+                pend &= pend - 1u;
+                b = V.row_ptr[i];
+                e = V.row_ptr[i + 1];
+            }
+            const uint32_t n = push_rows(V, C, __ballot(has), b, e, lane);
+            if (lane == 0) w.pushed += n;
+        }
+    }
+}
+
 // Bits of agents in the heavy region are set and cleared one agent at a time
 // (their words are shared by tiles that run concurrently).
 __device__ __forceinline__ void put_bits(SS_G uint64_t *nb, int64_t word, uint64_t set, uint64_t clr) {
@@ -700,6 +994,10 @@ __device__ __forceinline__ void flush_dir(Dir &d, unsigned long long *acc_next, 
     d = Dir{};
 }
 
+// LDS queue of the light tiles: the wide tile's for 16 agents per lane.
+template <int A> struct LightQueue { typedef ScanQueue type; };
+template <> struct LightQueue<SS_WIDE_A> { typedef WideQueue type; };
+
 // Launch step k of a tick loop: every live replica r runs tick reps[r].tick0 + k.
 // Blocks stride over the tile list (hub tiles first, then heavy, then light),
 // so every block gets a share of the expensive tiles; no queue, no atomics.
@@ -715,7 +1013,7 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
     __shared__ int s_cnt;
     __shared__ long long s_pos;
     __shared__ unsigned long long s_red[2 * SS_BLOCK / 64];
-    __shared__ ScanQueue Q;
+    __shared__ typename LightQueue<A>::type Q;
     Work w;
     Dir dir;
     int cached = -1;
@@ -811,8 +1109,12 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
         // agent that draws and the same states in both buffers (block-uniform)
         if (C->tcur && ((const SS_G uint8_t *)C->tcur)[tix] != V.tag) continue;
         if (threadIdx.x == 0) w.swept += (uint32_t)(end - begin);
-        if (kind == SS_TILE_LIGHT)
-            light_tile<A>(V, C, begin, end, tix, w, dir, Q);
+        if (kind == SS_TILE_LIGHT) {
+            if constexpr (A == SS_WIDE_A)
+                light_wide(V, C, begin, end, tix, w, dir, Q);
+            else
+                light_tile<A>(V, C, begin, end, tix, w, dir, Q);
+        }
         else if (kind == SS_TILE_HEAVY)
             heavy_tile(V, C, begin, end, tix, w, dir);
         else
@@ -1113,6 +1415,9 @@ hipError_t ss_launch_tick(const ss_rep_dev *reps, const ss_tile *tiles, int n_ti
     case 8:
         hipLaunchKernelGGL(tick_kernel<8>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k, counters);
         break;
+    case 16:
+        hipLaunchKernelGGL(tick_kernel<16>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k, counters);
+        break;
     default:
         return hipErrorInvalidValue;
     }
@@ -1157,6 +1462,7 @@ hipError_t ss_occupancy_tick(int apl, int *blocks_per_cu) {
     case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<2>, SS_BLOCK, 0);
     case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<4>, SS_BLOCK, 0);
     case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<8>, SS_BLOCK, 0);
+    case 16: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<16>, SS_BLOCK, 0);
     default: return hipErrorInvalidValue;
     }
 }

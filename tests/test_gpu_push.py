@@ -17,11 +17,13 @@ def say(*a):
     print("[case]", *a, flush=True)
 
 
-@pytest.fixture(scope="module")
-def engines():
+@pytest.fixture(scope="module", params=[4, 16], ids=["apl4", "apl16"])
+def engines(request):
     es = [Engine(0) for _ in RULES]
     for e, r in zip(es, RULES):
         e.set_push(r)
+        e.set_tiling(64, request.param, 0, True)
+    say("agents per lane", request.param)
     yield es
     for e in es:
         e.close()

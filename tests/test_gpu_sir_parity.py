@@ -54,7 +54,8 @@ def test_er_sir_every_tick(eng):
 @pytest.mark.parametrize("hub,heavy,apl,graph_loop", [(1024, 64, 1, True), (16, 8, 2, True),
                                                       (1 << 30, 1 << 30, 4, False), (64, 16, 1, False),
                                                       (32, 32, 4, True), (8, 8, 1, True), (1024, 64, 8, True),
-                                                      (1 << 30, 1 << 30, 8, True)])
+                                                      (1 << 30, 1 << 30, 8, True), (1024, 64, 16, True),
+                                                      (1 << 30, 1 << 30, 16, True), (8, 8, 16, False)])
 def test_ba_launch_shapes(eng, hub, heavy, apl, graph_loop):
     """BA(50k, 5), 60 ticks, in one captured loop: every launch shape gives the oracle's states."""
     w = workloads.ba_sir(n=50_000, m=5, beta=0.06, ticks=60, seed=3)
@@ -117,7 +118,8 @@ def test_edge_cases(eng):
     cases.append(("chain_beta1", graphs.from_edges(101, np.arange(100), np.arange(1, 101)), 1.0, 1.0))
     g = graphs.erdos_renyi(1000, 0.01, 9)
     cases.append(("er_gamma1", g, 0.2, 1.0))
-    for (name, g, beta, gamma), (hub, heavy, apl) in zip(cases * 2, [(1024, 64, 1)] * len(cases) + [(8, 8, 2)] * len(cases)):
+    shapes = [(1024, 64, 1)] * len(cases) + [(8, 8, 2)] * len(cases) + [(1 << 30, 1 << 30, 16)] * len(cases)
+    for (name, g, beta, gamma), (hub, heavy, apl) in zip(cases * 3, shapes):
         eng.set_tiling(heavy, apl, 0, True, hub_degree=hub)
         for init in ("one", "all"):
             st = np.zeros(g.n, np.int8)
