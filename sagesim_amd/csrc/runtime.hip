@@ -165,57 +165,94 @@ void drop_graphs(ss_ctx *ctx) {
 
 // Tiles of every live replica: one HUB tile per row of degree >= hub_deg at the
 // front, HEAVY tiles of SS_HEAVY_TILE agents up to the first 64-aligned slot
-// past the rows of degree >= heavy_deg, LIGHT tiles of 256 * apl agents for
-// the rest (light tiles start 64-aligned: they write whole bitmap words, the
-// hub and heavy tiles set single bits).  Blocks stride over the list, hubs
-// first, so each block starts with its share of the expensive tiles.
+// past the rows of degree >= heavy_deg, LIGHT tiles for the rest (light tiles
+// start 64-aligned: they write whole bitmap words, the hub and heavy tiles set
+// single bits).  A light tile holds 256 * apl agents (4 * 1024 with 16 agents
+// per lane), fewer where its rows are longer than the replica's average light
+// row: a tile of rows of degree d > dbar holds (dbar / d)^1.5 of that, in
+// steps of 64 agents, so that the tiles at the long-row end cost about as much
+// as the others.  Blocks stride over the list, hubs first, so each block
+// starts with its share of the expensive tiles.
 int build_tiles(ss_ctx *ctx) {
     std::vector<ss_tile> hubs, heavy, light;
-    const int64_t cap = (int64_t)SS_BLOCK * ctx->apl;
-    struct Pos { int64_t hub, heavy, light; };
-    std::vector<Pos> pos(ctx->reps.size(), Pos{0, 0, 0});
+    int bpc = 0;
+    SS_HIP(ctx, ss_occupancy_tick(ctx->apl, &bpc));
+    if (bpc < 1) bpc = 1;
+    const int g = ctx->grid_blocks > 0 ? ctx->grid_blocks : ctx->num_cus * bpc;
+    const int64_t cap = (int64_t)(ctx->apl == 16 ? 4 * 1024 : SS_BLOCK * ctx->apl);
+    struct Pos { int64_t hub, heavy, light, lblocks; };
+    std::vector<Pos> pos(ctx->reps.size(), Pos{0, 0, 0, 0});
+    int64_t n_lblocks = 0;                       // tag slots of light agents, all replicas
     for (int32_t ri = 0; ri < (int32_t)ctx->reps.size(); ++ri) {
         ss_replica &r = ctx->reps[ri];
         if (!r.live || r.n_total == 0) continue;
         const int64_t H = heavy_prefix(r, ctx->heavy_deg), Hh = heavy_prefix(r, ctx->hub_deg);
         if (H < 0 || Hh < 0) SS_FAIL(ctx, SS_ERR_INVALID, "degree threshold without a prefix table");
-        const int64_t H64 = std::min<int64_t>((H + 63) / 64 * 64, r.n_total);
+        const int64_t nt = r.n_total;
+        const int64_t H64 = std::min<int64_t>((H + 63) / 64 * 64, nt);
         // hub count rounded up to 8: the 8 agents of a heavy wavefront then
         // always share one bitmap word
         const int64_t nh = std::min((Hh + 7) / 8 * 8, H64);
-        pos[ri] = Pos{(int64_t)hubs.size(), (int64_t)heavy.size(), (int64_t)light.size()};
-        for (int64_t b = 0; b < nh; ++b) hubs.push_back({ri, SS_TILE_HUB, b, b + 1, 0});
+        pos[ri] = Pos{(int64_t)hubs.size(), (int64_t)heavy.size(), (int64_t)light.size(), n_lblocks};
+        for (int64_t b = 0; b < nh; ++b) hubs.push_back({ri, SS_TILE_HUB, b, b + 1, 0, 0});
         for (int64_t b = nh; b < H64; b += SS_HEAVY_TILE)
-            heavy.push_back({ri, SS_TILE_HEAVY, b, std::min<int64_t>(b + SS_HEAVY_TILE, H64), 0});
-        for (int64_t b = H64; b < r.n_total; b += cap)
-            light.push_back({ri, SS_TILE_LIGHT, b, std::min<int64_t>(b + cap, r.n_total), 0});
+            heavy.push_back({ri, SS_TILE_HEAVY, b, std::min<int64_t>(b + SS_HEAVY_TILE, H64), 0, 0});
+        // CSR entries before agent i (i a multiple of 64, or n_total)
+        auto E = [&](int64_t i) { return i >= nt ? r.edge64.back() : r.edge64[(size_t)(i / 64)]; };
+        const double dbar = H64 < nt ? (double)(E(nt) - E(H64)) / (double)(nt - H64) : 0.0;
+        for (int64_t b = H64; b < nt;) {
+            const int64_t g1 = std::min<int64_t>(b + 64, nt);
+            const double d = (double)(E(g1) - E(b)) / (double)(g1 - b);
+            int64_t a = cap;
+            if (dbar > 0.0 && d > dbar) a = (int64_t)((double)cap * std::pow(dbar / d, 1.5));
+            a = std::max<int64_t>(64, std::min(a, cap) / 64 * 64);
+            const int64_t e = std::min(b + a, nt);
+            light.push_back({ri, SS_TILE_LIGHT, b, e, 0, 0});
+            b = e;
+        }
+        n_lblocks += ((nt - H64) + (1 << SS_LTAG_SHIFT) - 1) >> SS_LTAG_SHIFT;
         r.dev.n_hub = nh;
         r.dev.h64 = H64;
         r.dev.light_cap = (int32_t)cap;
+    }
+    const int64_t nhub = (int64_t)hubs.size(), nheavy = (int64_t)heavy.size(), n_block = nhub + nheavy;
+    // tag slots: a hub or heavy tile's own index; light agents by blocks after them
+    for (auto &t : hubs) t.tag_lo = t.tag_hi = (int32_t)(pos[t.rep].hub + t.begin);
+    for (auto &t : heavy)
+        t.tag_lo = t.tag_hi = (int32_t)(nhub + pos[t.rep].heavy + (t.begin - ctx->reps[t.rep].dev.n_hub) / SS_HEAVY_TILE);
+    for (auto &t : light) {
+        const int64_t l0 = n_block + pos[t.rep].lblocks, h64 = ctx->reps[t.rep].dev.h64;
+        t.tag_lo = (int32_t)(l0 + ((t.begin - h64) >> SS_LTAG_SHIFT));
+        t.tag_hi = (int32_t)(l0 + ((t.end - 1 - h64) >> SS_LTAG_SHIFT));
     }
     std::vector<ss_tile> all;
     all.reserve(hubs.size() + heavy.size() + light.size());
     all.insert(all.end(), hubs.begin(), hubs.end());
     all.insert(all.end(), heavy.begin(), heavy.end());
     all.insert(all.end(), light.begin(), light.end());
-    if ((int64_t)all.size() > INT32_MAX / 2) SS_FAIL(ctx, SS_ERR_RANGE, "too many tiles");
+    const int64_t n_slots = n_block + n_lblocks;
+    if ((int64_t)all.size() > INT32_MAX / 2 || n_slots > INT32_MAX / 2) SS_FAIL(ctx, SS_ERR_RANGE, "too many tiles");
     SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if ((int64_t)all.size() > ctx->tiles_cap) {
         dfree(ctx->d_tiles);
-        dfree(ctx->d_tact[0]);
-        dfree(ctx->d_tact[1]);
         ctx->d_tiles = nullptr;
-        ctx->d_tact[0] = ctx->d_tact[1] = nullptr;
         int64_t nc = std::max<int64_t>((int64_t)all.size() * 5 / 4, 1024);
         SS_HIP(ctx, dalloc(ctx, nullptr, &ctx->d_tiles, (size_t)nc));
+        ctx->tiles_cap = nc;
+    }
+    if (n_slots > ctx->slots_cap) {
+        dfree(ctx->d_tact[0]);
+        dfree(ctx->d_tact[1]);
+        ctx->d_tact[0] = ctx->d_tact[1] = nullptr;
+        int64_t nc = std::max<int64_t>(n_slots * 5 / 4, 1024);
         SS_HIP(ctx, dalloc(ctx, nullptr, &ctx->d_tact[0], (size_t)nc));
         SS_HIP(ctx, dalloc(ctx, nullptr, &ctx->d_tact[1], (size_t)nc));
-        ctx->tiles_cap = nc;
+        ctx->slots_cap = nc;
     }
     // tile tags start cleared: a replica that was pushing with tile tags runs
     // every tile at its next tick (its agent tags stay valid)
-    SS_HIP(ctx, hipMemsetAsync(ctx->d_tact[0], 0, (size_t)ctx->tiles_cap, ctx->stream));
-    SS_HIP(ctx, hipMemsetAsync(ctx->d_tact[1], 0, (size_t)ctx->tiles_cap, ctx->stream));
+    SS_HIP(ctx, hipMemsetAsync(ctx->d_tact[0], 0, (size_t)ctx->slots_cap, ctx->stream));
+    SS_HIP(ctx, hipMemsetAsync(ctx->d_tact[1], 0, (size_t)ctx->slots_cap, ctx->stream));
     for (auto &r : ctx->reps)
         if (r.live && r.dec) SS_HIP(ctx, ss_launch_dec_demote(r.dec, ctx->stream));
     if (!all.empty())
@@ -223,19 +260,14 @@ int build_tiles(ss_ctx *ctx) {
                                    ctx->stream));
     SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->n_tiles = (int)all.size();
-    const int64_t nhub = (int64_t)hubs.size(), nheavy = (int64_t)heavy.size();
     for (int32_t ri = 0; ri < (int32_t)ctx->reps.size(); ++ri) {
         ss_replica &r = ctx->reps[ri];
         r.dev.tact[0] = ctx->d_tact[0];
         r.dev.tact[1] = ctx->d_tact[1];
         r.dev.hub0 = (int32_t)pos[ri].hub;
         r.dev.heavy0 = (int32_t)(nhub + pos[ri].heavy);
-        r.dev.light0 = (int32_t)(nhub + nheavy + pos[ri].light);
+        r.dev.light0 = (int32_t)(n_block + pos[ri].lblocks);
     }
-    int bpc = 0;
-    SS_HIP(ctx, ss_occupancy_tick(ctx->apl, &bpc));
-    if (bpc < 1) bpc = 1;
-    int g = ctx->grid_blocks > 0 ? ctx->grid_blocks : ctx->num_cus * bpc;
     ctx->grid = std::max(1, std::min(g, std::max(ctx->n_tiles, 1)));
     // the block of a replica's first tile writes its per-tick flags
     for (auto &r : ctx->reps) r.dev.lead_tile = -1;
@@ -446,6 +478,10 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
     r.n_local = nl; r.n_ghost = ng; r.n_total = nt; r.nnz = nnz;
     r.n_words = (nt + 63) / 64;
     r.max_deg = max_deg;
+    // CSR entries before agents 0, 64, 128, ... and before n_total (light tile sizes)
+    r.edge64.resize((size_t)((nt + 63) / 64 + 1));
+    for (int64_t j = 0; j * 64 < nt; ++j) r.edge64[(size_t)j] = nl > 0 ? sp->row_ptr[std::min(j * 64, nl)] : 0;
+    r.edge64.back() = nnz;
     // heavy prefix lengths for the tiling thresholds the autotuner may pick
     for (int32_t thr : SS_THRESHOLDS) {
         int64_t H = 0;

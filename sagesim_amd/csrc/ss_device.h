@@ -47,14 +47,18 @@ struct ss_rep_dev {
     unsigned long long *acc;
     uint32_t push_num;          // push when 16 * sum_deg(I) <= push_num * #S (0: never)
     int32_t lead_tile;          // the tile whose block writes the replica's per-tick flags
-    // Tile of an agent (for the tile tags of the push direction): agents
-    // [0, n_hub) have a HUB tile each from hub0 on, [n_hub, h64) HEAVY tiles of
-    // SS_HEAVY_TILE agents from heavy0 on, the rest LIGHT tiles of light_cap
-    // agents (a power of two) from light0 on.  tact[t & 1][tile] == tag(t): the tile runs at t.
+    // Tag slot of an agent (for the tile tags of the push direction): agents
+    // [0, n_hub) have a HUB tile each, whose index is its slot, from hub0 on;
+    // [n_hub, h64) HEAVY tiles of SS_HEAVY_TILE agents, slots from heavy0 on;
+    // the rest blocks of 2^SS_LTAG_SHIFT agents, slots from light0 on (a light
+    // tile runs when one of the blocks it overlaps is tagged).
+    // tact[t & 1][slot] == tag(t): the slot's agents may change at tick t.
     uint8_t *tact[2];
     int64_t n_hub, h64;
     int32_t hub0, heavy0, light0, light_cap;
 };
+
+#define SS_LTAG_SHIFT 10
 
 #define SS_ACC_SLOTS 16
 #define SS_ACC_WORDS (3 * SS_ACC_SLOTS * 2)
@@ -75,7 +79,7 @@ struct ss_tile {
     int32_t kind;
     int64_t begin;
     int64_t end;      // exclusive; end - begin <= tile capacity
-    int64_t pad;
+    int32_t tag_lo, tag_hi;   // tag slots the tile's agents map to (inclusive)
 };
 
 // Philox-4x32-10 (Salmon et al., SC'11), word 0 only.

@@ -94,6 +94,7 @@ struct ss_replica {
     int64_t max_deg = 0;
     std::vector<int64_t> heavy_prefix_deg;      // thresholds
     std::vector<int64_t> heavy_prefix_len;      // rows at the front with deg >= threshold
+    std::vector<int64_t> edge64;                // CSR entries before agents 0, 64, 128, ... and n_total
     ss_rep_dev dev{};                           // device descriptor mirror (pointers filled)
     size_t bytes = 0;                           // device bytes held
 };
@@ -111,12 +112,13 @@ struct ss_ctx {
     ss_tile *d_tiles = nullptr;
     uint8_t *d_tact[2] = {nullptr, nullptr};   // tile tags of the push direction by tick parity
     int64_t tiles_cap = 0;
+    int64_t slots_cap = 0;                      // tag slots allocated in d_tact
     int n_tiles = 0;
     bool tiles_dirty = true;
     // tiling (autotuned by the host library)
     int hub_deg = 1024;
-    int heavy_deg = 64;
-    int apl = 4;
+    int heavy_deg = 128;
+    int apl = 16;
     int grid_blocks = 0;                        // 0: persistent grid from occupancy
     int use_graph = 1;
     int grid = 0;

@@ -72,7 +72,7 @@ struct Cold {
     uint32_t *pcnt;              // infectious-neighbour counts for the next tick
     uint8_t *tnext;              // tile tags for the next tick
     int64_t n_hub, h64;          // agent -> tile mapping of the replica
-    int32_t hub0, heavy0, light0, light_shift;   // light tiles hold 2^light_shift agents
+    int32_t hub0, heavy0, light0;          // tag slots (see ss_rep_dev)
     uint8_t ptag;                // tile tag of the next tick
 };
 
@@ -84,21 +84,21 @@ struct Pusher {
     SS_G uint32_t *pcnt;
     SS_G uint8_t *tnext;
     int64_t n_hub, h64;
-    int32_t hub0, heavy0, light0, light_shift;
+    int32_t hub0, heavy0, light0;
     uint8_t ptag;
     __device__ __forceinline__ explicit Pusher(const LCold *C)
         : pcnt((SS_G uint32_t *)C->pcnt), tnext((SS_G uint8_t *)C->tnext), n_hub(C->n_hub), h64(C->h64),
-          hub0(C->hub0), heavy0(C->heavy0), light0(C->light0), light_shift(C->light_shift), ptag(C->ptag) {}
-    // tile of agent v (see ss_rep_dev)
-    __device__ __forceinline__ int64_t tile_of(int64_t v) const {
+          hub0(C->hub0), heavy0(C->heavy0), light0(C->light0), ptag(C->ptag) {}
+    // tag slot of agent v (see ss_rep_dev)
+    __device__ __forceinline__ int64_t slot_of(int64_t v) const {
         if (v < n_hub) return hub0 + v;
         if (v < h64) return heavy0 + ((v - n_hub) >> SS_HEAVY_SHIFT);
-        return light0 + ((v - h64) >> light_shift);
+        return light0 + ((v - h64) >> SS_LTAG_SHIFT);
     }
     // one more infectious neighbour for agent c next tick; its tile runs then
     __device__ __forceinline__ void one(int32_t c) const {
         __hip_atomic_fetch_add(pcnt + c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tnext[tile_of(c)] = ptag;
+        tnext[slot_of(c)] = ptag;
     }
     // entries [b, e) of col, by `width` threads from `me` on, four loads in
     // flight per thread
@@ -274,6 +274,26 @@ __device__ __forceinline__ bool scan_wave(const TickView V, const LCold *C, int6
     return false;
 }
 
+// Scans of the rows of the lanes flagged in `rows` (each lane's own b, e, m),
+// one after another by the whole wavefront; the flagged lanes receive inf and
+// first.  Called by the whole wavefront.
+__device__ __forceinline__ void scan_rows(const TickView &V, const LCold *C, uint64_t rows, int64_t b, int64_t e,
+                                          uint32_t m, int lane, bool &inf, int32_t &first, Work &w) {
+    while (rows) {                                   // wave-uniform
+        const int l = __ffsll((unsigned long long)rows) - 1;
+        rows &= rows - 1;
+        const int64_t bl = ((int64_t)__shfl((int32_t)(b >> 32), l) << 32) | (uint32_t)__shfl((int32_t)b, l);
+        const int64_t el = ((int64_t)__shfl((int32_t)(e >> 32), l) << 32) | (uint32_t)__shfl((int32_t)e, l);
+        const uint32_t ml = (uint32_t)__shfl((int32_t)m, l);
+        int32_t fl = -1;
+        const bool il = scan_wave(V, C, bl, el, ml, lane, fl, w);
+        if (lane == l) {
+            inf = il;
+            first = fl;
+        }
+    }
+}
+
 // Neighbour scans of a light tile, gathered in LDS so that the lanes of a
 // wavefront scan concurrently instead of each lane walking its own agents in
 // turn.  Each wavefront owns the queue section of its own agents.
@@ -283,6 +303,14 @@ struct ScanQueue {
     uint16_t slot[SS_LIGHT_MAX];   // agent - tile begin
     uint8_t hit[SS_LIGHT_MAX];     // scan result by slot
 };
+
+// The value, hidden from the optimiser: lane-derived values computed from it
+// inside a tile function are not hoisted out of the tile loop, where they
+// would stay live (or be spilled) across every kind of tile.
+__device__ __forceinline__ int opaque(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
 
 // Orders a wavefront's LDS writes before its later LDS reads by other lanes
 // (LDS operations of one wavefront complete in order).
@@ -304,7 +332,7 @@ __device__ __forceinline__ void wave_sync() {
 #define SS_QHIST 0x8000u
 template <int A>
 __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int64_t begin, int64_t end,
-                                           int64_t tix, Work &w, Dir &dir, ScanQueue &Q) {
+                                           int32_t tag_lo, int32_t tag_hi, Work &w, Dir &dir, ScanQueue &Q) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t g0 = begin + (int64_t)wv * 64 * A;
     const int sec = wv * 64 * A;                     // this wavefront's queue section
@@ -430,7 +458,8 @@ __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int
     // next tick's tile tag: a state changed (the other buffer is stale), an
     // agent will draw or is infectious (its row is pushed every tick);
     // candidates get theirs from the rows pushed below
-    if (V.push_next && __ballot(busy) && lane == 0) ((SS_G uint8_t *)C->tnext)[tix] = C->ptag;
+    if (V.push_next && __ballot(busy) && lane == 0)
+        for (int64_t q = tag_lo; q <= tag_hi; ++q) ((SS_G uint8_t *)C->tnext)[q] = C->ptag;
     if (V.push_next && __ballot(pmask != 0u)) {
 #pragma unroll 1
         for (int a = 0; a < A; ++a) {
@@ -448,8 +477,8 @@ __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int
     }
 }
 
-// Wide LIGHT tile: each wavefront owns a chunk of 1024 consecutive agents and
-// lane l the 16 agents [16 l, 16 l + 16) of it, so states, counts, degrees and
+// Wide LIGHT tile: each wavefront owns a chunk of up to 1024 consecutive agents
+// and lane l the 16 agents [16 l, 16 l + 16) of it, so states, counts, degrees and
 // RNG keys arrive as 16-byte vector loads, one round trip for the chunk.  The
 // agents that draw (susceptible candidates, agents with a spontaneous
 // transition) are packed into the wavefront's LDS queue, so the draws run on
@@ -458,7 +487,7 @@ __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int
 // whole wavefront.  Results return through a byte per chunk slot.
 #define SS_WIDE_A 16
 #define SS_WIDE_CHUNK (64 * SS_WIDE_A)
-#define SS_WIDE_LONG 64          // rows longer than this are scanned by the whole wavefront
+#define SS_WIDE_SHORT 64         // rows longer than this are scanned by the whole wavefront
 struct alignas(16) WideQueue {
     // draw entry: x = RNG key (pull) or infectious-neighbour count (push);
     //   y = slot | transition << 10 | (state & 3) << 11 | deg16 << 16 (deg16 only on pull)
@@ -500,11 +529,14 @@ __device__ __forceinline__ int64_t deg_of(const TickView &V, int64_t i, uint32_t
     return d16 == 65535u ? V.row_ptr[i + 1] - V.row_ptr[i] : (int64_t)d16;
 }
 
-__device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int64_t begin, int64_t end, int64_t tix,
-                                           Work &w, Dir &dir, WideQueue &Q) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t c0 = begin + (int64_t)wv * SS_WIDE_CHUNK;      // chunk start, a multiple of 64
+__device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int64_t begin, int64_t end, Work &w,
+                                           Dir &dir, WideQueue &Q) {
+    const int lane = opaque((int)threadIdx.x & 63), wv = threadIdx.x >> 6;
+    // the wavefront's chunk: a quarter of the tile rounded up to 64 agents (at most 1024)
+    const int64_t chunk = ((end - begin + 3) / 4 + 63) & ~(int64_t)63;
+    const int64_t c0 = begin + (int64_t)wv * chunk;
     if (c0 >= end) return;                                        // wave-uniform: no block barrier below
+    if (end > c0 + chunk) end = c0 + chunk;
     uint2 *const E = Q.e[wv];
     uint8_t *const res = Q.res[wv];
     const uint64_t below = (1ull << lane) - 1ull;
@@ -548,7 +580,7 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             if ((a[4 * q] | a[4 * q + 1] | a[4 * q + 2] | a[4 * q + 3]) != 0u)
-                st4(V.cnt + i0 + 4 * q, make_uint4(0u, 0u, 0u, 0u));
+                st4(V.cnt + i0 + 4 * q, make_uint4(0u, 0u, 0u, (uint32_t)opaque(0)));
     }
     // states with a spontaneous transition, as a bit set over state codes 0..3
     uint32_t tbits = 0u;
@@ -657,25 +689,12 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
                 const uint32_t d = y >> 16;
                 b = V.row_ptr[i];
                 e = d == 65535u ? V.row_ptr[i + 1] : b + (int64_t)d;
-                lng = e - b > SS_WIDE_LONG;
+                lng = e - b > SS_WIDE_SHORT;
             }
             bool inf = false;
             int32_t first = -1;
             if (q < nsq && !lng) inf = scan_lane(V, C, b, e, m, first, w);
-            uint64_t lm = __ballot(lng);
-            while (lm) {
-                const int l = __ffsll((unsigned long long)lm) - 1;
-                lm &= lm - 1;
-                const int64_t bl = ((int64_t)__shfl((int32_t)(b >> 32), l) << 32) | (uint32_t)__shfl((int32_t)b, l);
-                const int64_t el = ((int64_t)__shfl((int32_t)(e >> 32), l) << 32) | (uint32_t)__shfl((int32_t)e, l);
-                const uint32_t ml = (uint32_t)__shfl((int32_t)m, l);
-                int32_t fl = -1;
-                const bool il = scan_wave(V, C, bl, el, ml, lane, fl, w);
-                if (lane == l) {
-                    inf = il;
-                    first = fl;
-                }
-            }
+            scan_rows(V, C, __ballot(lng), b, e, m, lane, inf, first, w);
             if (q < nsq && inf) {
                 const uint32_t slot = y & 1023u;
                 const int64_t i = c0 + slot;
@@ -726,7 +745,10 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
         // next tick's tile tag: a state changed (the other buffer is stale), an
         // agent will draw or is infectious (its row is pushed every tick);
         // candidates get theirs from the rows pushed below
-        if (__ballot(busy || pend != 0u) && lane == 0) ((SS_G uint8_t *)C->tnext)[tix] = C->ptag;
+        if (__ballot(busy || pend != 0u) && lane == 0) {
+            const Pusher P(C);
+            for (int64_t q = P.slot_of(c0); q <= P.slot_of(end - 1); ++q) P.tnext[q] = P.ptag;
+        }
         while (__ballot(pend != 0u)) {
             const bool has = pend != 0u;
             int64_t b = 0, e = 0;
@@ -794,16 +816,12 @@ __device__ __forceinline__ void heavy_tile(const TickView V, const LCold *C, int
             }
         }
     }
-    uint64_t pend = __ballot(need);
-    while (pend) {
-        const int l = __ffsll((unsigned long long)pend) - 1;
-        pend &= pend - 1;
-        const int64_t bl = ((int64_t)__shfl((int32_t)(b >> 32), l) << 32) | (uint32_t)__shfl((int32_t)b, l);
-        const int64_t el = ((int64_t)__shfl((int32_t)(e >> 32), l) << 32) | (uint32_t)__shfl((int32_t)e, l);
-        const uint32_t ml = (uint32_t)__shfl((int32_t)m, l);
+    const uint64_t pend = __ballot(need);
+    if (pend) {                                      // wave-uniform
+        bool inf = false;
         int32_t first = -1;
-        const bool inf = scan_wave(V, C, bl, el, ml, lane, first, w);
-        if (lane == l && inf) {
+        scan_rows(V, C, pend, b, e, m, lane, inf, first, w);
+        if (need && inf) {
             ns = V.s_target;
             if (C->history) ss_hist_append(C->history + i * SS_HIST_W, first);
         }
@@ -1004,7 +1022,9 @@ template <> struct LightQueue<SS_WIDE_A> { typedef WideQueue type; };
 // Four waves per SIMD are asked for: the few values that then live in scratch
 // are read once per tile, while a fifth of the latency hiding would be lost
 // at three waves.
-template <int A>
+// CNT: the work counters are kept (a separate instantiation, so that the
+// counting registers cost nothing when counting is off).
+template <int A, bool CNT>
 __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void tick_kernel(const ss_rep_dev *__restrict__ reps,
                                                         const ss_tile *__restrict__ tiles, int n_tiles, int k,
                                                         unsigned long long *counters) {
@@ -1023,7 +1043,7 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
     if ((int)blockIdx.x < n_tiles) nt = tiles[blockIdx.x];
     for (int tix = blockIdx.x; tix < n_tiles; tix += gridDim.x) {
         // the next tile's descriptor is loaded while this one runs
-        const int32_t rep = nt.rep, kind = nt.kind;
+        const int32_t rep = nt.rep, kind = nt.kind, tag_lo = nt.tag_lo, tag_hi = nt.tag_hi;
         const int64_t begin = nt.begin, end = nt.end;
         if (tix + (int)gridDim.x < n_tiles) nt = tiles[tix + gridDim.x];
         if (rep != cached) {                         // block-uniform
@@ -1047,7 +1067,6 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
                     C->hub0 = R->hub0;
                     C->heavy0 = R->heavy0;
                     C->light0 = R->light0;
-                    C->light_shift = 31 - __clz(R->light_cap);
                     C->ptag = ss_front_tag(t1);
                 }
                 if (threadIdx.x < 4) {
@@ -1085,7 +1104,7 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
                     atomicAdd(&nx[1], c);
                     unsigned long long *z = acc_slot(R, t + 2);
                     for (int q = 0; q < SS_ACC_SLOTS * 2; ++q) z[q] = 0ull;
-                    if (counters && dec != 0u) atomicAdd(&counters[5], 1ull);
+                    if (CNT && dec != 0u) atomicAdd(&counters[5], 1ull);
                 }
                 V.last = last;
                 V.s_code = R->s_code;
@@ -1107,13 +1126,17 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
         if (!live) continue;                         // replica finished: block-uniform
         // push tick with tile tags: a tile without a tag has no candidate, no
         // agent that draws and the same states in both buffers (block-uniform)
-        if (C->tcur && ((const SS_G uint8_t *)C->tcur)[tix] != V.tag) continue;
+        if (C->tcur) {
+            bool run = false;
+            for (int32_t q = tag_lo; q <= tag_hi; ++q) run |= ((const SS_G uint8_t *)C->tcur)[q] == V.tag;
+            if (!run) continue;
+        }
         if (threadIdx.x == 0) w.swept += (uint32_t)(end - begin);
         if (kind == SS_TILE_LIGHT) {
             if constexpr (A == SS_WIDE_A)
-                light_wide(V, C, begin, end, tix, w, dir, Q);
+                light_wide(V, C, begin, end, w, dir, Q);
             else
-                light_tile<A>(V, C, begin, end, tix, w, dir, Q);
+                light_tile<A>(V, C, begin, end, tag_lo, tag_hi, w, dir, Q);
         }
         else if (kind == SS_TILE_HEAVY)
             heavy_tile(V, C, begin, end, tix, w, dir);
@@ -1121,7 +1144,7 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
             hub_tile(V, C, begin, tix, w, dir, &s_cnt, &s_pos);
     }
     if (cached >= 0) flush_dir(dir, C->acc_next, s_red);
-    if (counters) flush_work(counters, w);
+    if constexpr (CNT) flush_work(counters, w);
 }
 
 // Infectious-neighbour counts of the current tick of one replica (cleared
@@ -1402,25 +1425,25 @@ inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b);
 hipError_t ss_launch_tick(const ss_rep_dev *reps, const ss_tile *tiles, int n_tiles, int k, int apl, int grid,
                           unsigned long long *counters, hipStream_t st) {
     if (n_tiles <= 0) return hipSuccess;
+#define SS_TICK_CASE(a)                                                                                     \
+    case a:                                                                                                 \
+        if (counters)                                                                                       \
+            hipLaunchKernelGGL((tick_kernel<a, true>), dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k, \
+                               counters);                                                                   \
+        else                                                                                                \
+            hipLaunchKernelGGL((tick_kernel<a, false>), dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, \
+                               k, counters);                                                                \
+        break;
     switch (apl) {
-    case 1:
-        hipLaunchKernelGGL(tick_kernel<1>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k, counters);
-        break;
-    case 2:
-        hipLaunchKernelGGL(tick_kernel<2>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k, counters);
-        break;
-    case 4:
-        hipLaunchKernelGGL(tick_kernel<4>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k, counters);
-        break;
-    case 8:
-        hipLaunchKernelGGL(tick_kernel<8>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k, counters);
-        break;
-    case 16:
-        hipLaunchKernelGGL(tick_kernel<16>, dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles, k, counters);
-        break;
+        SS_TICK_CASE(1)
+        SS_TICK_CASE(2)
+        SS_TICK_CASE(4)
+        SS_TICK_CASE(8)
+        SS_TICK_CASE(16)
     default:
         return hipErrorInvalidValue;
     }
+#undef SS_TICK_CASE
     return hipGetLastError();
 }
 
@@ -1458,11 +1481,11 @@ hipError_t ss_launch_symcheck(const int64_t *row_ptr, const int32_t *col_idx, in
 
 hipError_t ss_occupancy_tick(int apl, int *blocks_per_cu) {
     switch (apl) {
-    case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<1>, SS_BLOCK, 0);
-    case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<2>, SS_BLOCK, 0);
-    case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<4>, SS_BLOCK, 0);
-    case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<8>, SS_BLOCK, 0);
-    case 16: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<16>, SS_BLOCK, 0);
+    case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<1, false>, SS_BLOCK, 0);
+    case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<2, false>, SS_BLOCK, 0);
+    case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<4, false>, SS_BLOCK, 0);
+    case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<8, false>, SS_BLOCK, 0);
+    case 16: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tick_kernel<16, false>, SS_BLOCK, 0);
     default: return hipErrorInvalidValue;
     }
 }

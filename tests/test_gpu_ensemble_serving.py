@@ -60,7 +60,7 @@ def test_ensemble_matches_standalone_and_oracle(eng):
         assert np.array_equal(batched[i], _oracle(prepared[i])), i
 
 
-@pytest.mark.parametrize("shape", [(1024, 64, 8), (16, 8, 2)])
+@pytest.mark.parametrize("shape", [(1024, 64, 8), (16, 8, 2), (4096, 128, 16), (1 << 30, 1 << 30, 16)])
 def test_ensemble_launch_shapes(eng, shape):
     hub, heavy, apl = shape
     eng.set_tiling(heavy, apl, 0, True, hub_degree=hub)
@@ -70,7 +70,7 @@ def test_ensemble_launch_shapes(eng, shape):
     for i, p in enumerate(prepared):
         assert np.array_equal(ens.states(i), _oracle(p)), i
     ens.remove()
-    eng.set_tiling(64, 4, 0, True)
+    eng.set_tiling()
 
 
 def test_serving_alone_equals_in_mix(eng):

@@ -169,3 +169,24 @@ def test_si_model_keeps_pushing(engines):
             assert first_diff(rep.states(), trace[t0 + 29]) is None, (t0, first_diff(rep.states(), trace[t0 + 29]))
         rep.remove()
     assert 0 < (trace[-1] == 1).sum() < w.graph.n
+
+
+@pytest.mark.parametrize("grid", [7, 16])
+def test_wide_tiles_shrunk_chunks(grid):
+    """16 agents per lane on a small grid: the light agents take several rounds
+    of the grid, so the chunk per wavefront shrinks below 1024 agents (tiles
+    of a size that is not a power of two); every rule still gives the
+    oracle's states."""
+    w = workloads.ba_sir(n=120_000, m=4, beta=0.04, ticks=90, seed=61)
+    _, trace = O.sir_run(O.sir_model(w.beta, w.gamma, w.seed), w.graph.row_ptr, w.graph.col_idx,
+                         np.arange(w.graph.n), w.state, w.ticks, trace=True)
+    say("shrunk chunks", grid)
+    for rule in RULES:
+        with Engine(0) as e:
+            e.set_push(rule)
+            e.set_tiling(64, 16, grid, True)
+            rep = Replica(e, w.graph, w.state, w.model, w.seed)
+            for t0 in range(0, w.ticks, 15):
+                e.simulate(15)
+                assert first_diff(rep.states(), trace[t0 + 14]) is None, (rule, t0, first_diff(rep.states(), trace[t0 + 14]))
+            rep.remove()

@@ -69,7 +69,7 @@ def test_ba_launch_shapes(eng, hub, heavy, apl, graph_loop):
     assert first_diff(got, want) is None, first_diff(got, want)
     assert rep.tick() == w.ticks
     rep.remove()
-    eng.set_tiling(64, 1, 0, True)
+    eng.set_tiling()
 
 
 def test_ba_1m_subset_200_ticks(eng):
@@ -137,7 +137,7 @@ def test_edge_cases(eng):
                 got = rep.states()
                 assert first_diff(got, trace[t]) is None, (name, init, t, first_diff(got, trace[t]))
             rep.remove()
-    eng.set_tiling(64, 1, 0, True)
+    eng.set_tiling()
 
 
 @pytest.mark.parametrize("hub,heavy", [(1024, 64), (16, 8)])
@@ -167,7 +167,7 @@ def test_history_and_ghosts(eng, hub, heavy):
     assert (hist != pw.history).any()
     assert n_ghost > 0
     rep.remove()
-    eng.set_tiling(64, 1, 0, True)
+    eng.set_tiling()
 
 
 def test_hash_map_against_dict(eng):
