@@ -167,12 +167,11 @@ void drop_graphs(ss_ctx *ctx) {
 // front, HEAVY tiles of SS_HEAVY_TILE agents up to the first 64-aligned slot
 // past the rows of degree >= heavy_deg, LIGHT tiles for the rest (light tiles
 // start 64-aligned: they write whole bitmap words, the hub and heavy tiles set
-// single bits).  A light tile holds 256 * apl agents (4 * 1024 with 16 agents
-// per lane), fewer where its rows are longer than the replica's average light
-// row: a tile of rows of degree d > dbar holds (dbar / d)^1.5 of that, in
-// steps of 64 agents, so that the tiles at the long-row end cost about as much
-// as the others.  Blocks stride over the list, hubs first, so each block
-// starts with its share of the expensive tiles.
+// single bits).  A light tile holds 256 * apl agents; with 16 agents per lane
+// it holds 4 * 1024, fewer where its rows are longer than the replica's
+// average light row: a tile of rows of degree d > dbar holds (dbar / d)^1.5
+// of that, in steps of 256 agents, so that the tiles at the long-row end cost
+// about as much as the others.  Blocks stride over the list (see below).
 int build_tiles(ss_ctx *ctx) {
     std::vector<ss_tile> hubs, heavy, light;
     int bpc = 0;
@@ -204,8 +203,10 @@ int build_tiles(ss_ctx *ctx) {
             const int64_t g1 = std::min<int64_t>(b + 64, nt);
             const double d = (double)(E(g1) - E(b)) / (double)(g1 - b);
             int64_t a = cap;
-            if (dbar > 0.0 && d > dbar) a = (int64_t)((double)cap * std::pow(dbar / d, 1.5));
-            a = std::max<int64_t>(64, std::min(a, cap) / 64 * 64);
+            if (ctx->apl == 16 && dbar > 0.0 && d > dbar) a = (int64_t)((double)cap * std::pow(dbar / d, 1.5));
+            // whole groups of 4 x 64 agents: the 4 wavefronts of a wide tile get equal chunks
+            const int64_t step = ctx->apl == 16 ? 256 : 64;
+            a = std::max<int64_t>(step, std::min(a, cap) / step * step);
             const int64_t e = std::min(b + a, nt);
             light.push_back({ri, SS_TILE_LIGHT, b, e, 0, 0});
             b = e;
@@ -216,7 +217,8 @@ int build_tiles(ss_ctx *ctx) {
         r.dev.light_cap = (int32_t)cap;
     }
     const int64_t nhub = (int64_t)hubs.size(), nheavy = (int64_t)heavy.size(), n_block = nhub + nheavy;
-    // tag slots: a hub or heavy tile's own index; light agents by blocks after them
+    // tag slots: one per hub or heavy tile (its place in the list before the
+    // rounds below are reordered); light agents by blocks after them
     for (auto &t : hubs) t.tag_lo = t.tag_hi = (int32_t)(pos[t.rep].hub + t.begin);
     for (auto &t : heavy)
         t.tag_lo = t.tag_hi = (int32_t)(nhub + pos[t.rep].heavy + (t.begin - ctx->reps[t.rep].dev.n_hub) / SS_HEAVY_TILE);
@@ -230,6 +232,12 @@ int build_tiles(ss_ctx *ctx) {
     all.insert(all.end(), hubs.begin(), hubs.end());
     all.insert(all.end(), heavy.begin(), heavy.end());
     all.insert(all.end(), light.begin(), light.end());
+    // block b runs tiles b, b + G, b + 2G, ... of the list; the list runs from
+    // the longest rows to the shortest, so every other round of G tiles is
+    // reversed: no block gets the most expensive tile of every round
+    const int64_t G = std::max<int64_t>(1, std::min<int64_t>(g, (int64_t)all.size()));
+    for (int64_t r0 = G; r0 < (int64_t)all.size(); r0 += 2 * G)
+        std::reverse(all.begin() + r0, all.begin() + std::min<int64_t>(r0 + G, (int64_t)all.size()));
     const int64_t n_slots = n_block + n_lblocks;
     if ((int64_t)all.size() > INT32_MAX / 2 || n_slots > INT32_MAX / 2) SS_FAIL(ctx, SS_ERR_RANGE, "too many tiles");
     SS_HIP(ctx, hipStreamSynchronize(ctx->stream));

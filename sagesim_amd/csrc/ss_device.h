@@ -82,6 +82,15 @@ struct ss_tile {
     int32_t tag_lo, tag_hi;   // tag slots the tile's agents map to (inclusive)
 };
 
+// a ^ b ^ c: one V_BITOP3_B32 (truth table 0x96) on the device.
+__host__ __device__ __forceinline__ uint32_t ss_xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 // Philox-4x32-10 (Salmon et al., SC'11), word 0 only.
 __host__ __device__ __forceinline__ uint32_t ss_philox4x32_w0(uint32_t c0, uint32_t c1, uint32_t c2,
                                                               uint32_t c3, uint32_t k0, uint32_t k1) {
@@ -89,9 +98,9 @@ __host__ __device__ __forceinline__ uint32_t ss_philox4x32_w0(uint32_t c0, uint3
     for (int r = 0; r < 10; ++r) {
         uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n0 = ss_xor3((uint32_t)(p1 >> 32), c1, k0);
         uint32_t n1 = (uint32_t)p1;
-        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n2 = ss_xor3((uint32_t)(p0 >> 32), c3, k1);
         uint32_t n3 = (uint32_t)p0;
         c0 = n0; c1 = n1; c2 = n2; c3 = n3;
         k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;

@@ -1017,8 +1017,9 @@ template <int A> struct LightQueue { typedef ScanQueue type; };
 template <> struct LightQueue<SS_WIDE_A> { typedef WideQueue type; };
 
 // Launch step k of a tick loop: every live replica r runs tick reps[r].tick0 + k.
-// Blocks stride over the tile list (hub tiles first, then heavy, then light),
-// so every block gets a share of the expensive tiles; no queue, no atomics.
+// Blocks stride over the tile list (block b takes tiles b, b + G, ...; the
+// host orders the list so that every block gets a share of the expensive
+// tiles); no queue, no atomics.
 // Four waves per SIMD are asked for: the few values that then live in scratch
 // are read once per tile, while a fifth of the latency hiding would be lost
 // at three waves.
@@ -1139,9 +1140,9 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
                 light_tile<A>(V, C, begin, end, tag_lo, tag_hi, w, dir, Q);
         }
         else if (kind == SS_TILE_HEAVY)
-            heavy_tile(V, C, begin, end, tix, w, dir);
+            heavy_tile(V, C, begin, end, tag_lo, w, dir);
         else
-            hub_tile(V, C, begin, tix, w, dir, &s_cnt, &s_pos);
+            hub_tile(V, C, begin, tag_lo, w, dir, &s_cnt, &s_pos);
     }
     if (cached >= 0) flush_dir(dir, C->acc_next, s_red);
     if constexpr (CNT) flush_work(counters, w);
