@@ -195,7 +195,7 @@ def main():
                     "full_scan_equivalent_GBs": round(full_scan / (launch_us * 1e-6) / 1e9, 2),
                     "launch_us": round(launch_us, 2), "push_ticks": push_ticks,
                     "plain_copy_GBs": None if copy_gbs is None else round(copy_gbs, 1),
-                    "kernel": f"tick_kernel<{args.apl}>"}
+                    "kernel": f"tick_kernel<{args.apl}, false>"}
         log(f"state-aware {per_launch / 1e6:.2f} MB/launch -> {achieved:.1f} GB/s; touched "
             f"{touched / args.ticks / 1e6:.2f} MB/launch; push ticks {push_ticks}")
 
