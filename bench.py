@@ -305,7 +305,7 @@ def serving_leg(eng, args):
     from sagesim_amd import Replica
     from sagesim_amd.ensemble import Prepared, run_standalone
     from sagesim_amd.serving import Server, serving_requests
-    eng.set_tiling(64, 8, 0, True, hub_degree=4096)
+    eng.set_tiling()
     arrivals = serving_requests(rate=args.serve_rate, duration=args.serve_seconds)
 
     def make(req):
