@@ -149,7 +149,7 @@ int init_acc(ss_ctx *ctx, ss_replica &r) {
         const int par = (int)(r.tick & 1);
         const uint32_t num = r.dev.push_ok ? ctx->push_num : 0u;
         for (int b = 0; b < 2; ++b)
-            SS_HIP(ctx, hipMemsetAsync(r.cnt[b], 0, sizeof(uint32_t) * (size_t)pad_agents(r.n_local), ctx->stream));
+            SS_HIP(ctx, hipMemsetAsync(r.cnt[b], 0, sizeof(uint16_t) * (size_t)pad_agents(r.n_local), ctx->stream));
         SS_HIP(ctx, ss_launch_count_init(r.row_ptr, r.col_idx, r.ibits[par], r.n_local, heavy_prefix(r, 64),
                                          r.cnt[par], r.dec + par, r.acc + slot * (SS_ACC_SLOTS * 2), num,
                                          ctx->num_cus * 8, ctx->stream));
@@ -450,7 +450,7 @@ int64_t ss_replica_footprint(int64_t n_local, int64_t n_ghost, int64_t nnz, int3
     int64_t words = (n_total + 63) / 64;
     int64_t b = 8 * (n_total + 1) + 4 * nnz + 6 * pad_agents(n_local) + 2 * pad_agents(n_total) + n_total +
                 2 * 8 * words + 4 * (std::min<int64_t>(max_deg, 1 << 24) + 1) + 8 * SS_ACC_WORDS +
-                (n_ghost == 0 ? 8 * pad_agents(n_local) + 8 : 0);
+                (n_ghost == 0 ? 4 * pad_agents(n_local) + 8 : 0);
     if (with_history) b += 2 * 4 * n_local * SS_HISTORY_WIDTH;
     return b;
 }
@@ -598,7 +598,8 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
     r.dev.cnt[1] = r.cnt[1];
     r.dev.dec = r.dec;
     r.dev.acc = r.acc;
-    r.dev.push_ok = r.symmetric && r.cnt[0] ? 1u : 0u;
+    // 16-bit counts: every count stays below 65536
+    r.dev.push_ok = r.symmetric && r.cnt[0] && max_deg < 65535 ? 1u : 0u;
     r.dev.lead_tile = -1;
     r.dev.seed_lo = (uint32_t)sp->seed;
     r.dev.seed_hi = (uint32_t)(sp->seed >> 32);

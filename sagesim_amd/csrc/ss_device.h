@@ -37,8 +37,10 @@ struct ss_rep_dev {
     // agents it leaves infectious (the CSR is symmetric) when it decides to,
     // and cleared by tick t as it reads them.  dec[t & 1] records the
     // decision: 0 = pull, 1 = push with tile tags, 2 = push, every tile runs.
-    // Null when !push_ok.
-    uint32_t *cnt[2];
+    // 16-bit counts (rows shorter than 65535 entries, checked at creation),
+    // added to as halves of aligned 32-bit words (ss_count_add).  Null when
+    // !push_ok.
+    uint16_t *cnt[2];
     uint32_t *dec;
     // Direction statistics, [tick % 3][slot][2]: [0] = sum of degrees of the
     // infectious agents, [1] = number of susceptible agents at that tick,
@@ -81,6 +83,13 @@ struct ss_tile {
     int64_t end;      // exclusive; end - begin <= tile capacity
     int32_t tag_lo, tag_hi;   // tag slots the tile's agents map to (inclusive)
 };
+
+// One more infectious neighbour for agent c: the 16-bit count is one half of
+// an aligned 32-bit word (no carry: a count never reaches 65536).
+__device__ __forceinline__ void ss_count_add(uint16_t *cnt, int64_t c) {
+    __hip_atomic_fetch_add((__attribute__((address_space(1))) uint32_t *)(cnt + (c & ~(int64_t)1)),
+                           1u << (16 * (int)(c & 1)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // a ^ b ^ c: one V_BITOP3_B32 (truth table 0x96) on the device.
 __host__ __device__ __forceinline__ uint32_t ss_xor3(uint32_t a, uint32_t b, uint32_t c) {

@@ -50,7 +50,7 @@ hipError_t ss_launch_stats(const int8_t *st, const int64_t *row_ptr, int64_t n_l
 // push decision of the current tick from its statistics acc_t, then (when
 // pushing) its infectious-neighbour counts (cleared before) from the bitmap
 hipError_t ss_launch_count_init(const int64_t *row_ptr, const int32_t *col_idx, const uint64_t *bits,
-                                int64_t n_local, int64_t heavy, uint32_t *cnt, uint32_t *dec,
+                                int64_t n_local, int64_t heavy, uint16_t *cnt, uint32_t *dec,
                                 const unsigned long long *acc_t, uint32_t push_num, int grid, hipStream_t s);
 // a push decision with tile tags (1) becomes "push, every tile runs" (2)
 hipError_t ss_launch_dec_demote(uint32_t *dec, hipStream_t s);
@@ -76,7 +76,7 @@ struct ss_replica {
     uint64_t *ibits[2] = {nullptr, nullptr};
     int32_t *history = nullptr, *init_history = nullptr;
     uint32_t *inf_T = nullptr;
-    uint32_t *cnt[2] = {nullptr, nullptr};     // push-direction infectious-neighbour counts [n_local] by tick parity
+    uint16_t *cnt[2] = {nullptr, nullptr};     // push-direction infectious-neighbour counts [n_local] by tick parity
     uint32_t *dec = nullptr;                   // push decisions [2] by tick parity
     unsigned long long *acc = nullptr;         // direction statistics [SS_ACC_WORDS]
     bool symmetric = false;                    // every entry (i, j) has its mirror (j, i)

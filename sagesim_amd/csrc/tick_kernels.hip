@@ -49,7 +49,7 @@ struct TickView {
     const SS_G uint64_t *cb;
     SS_G uint64_t *nb;
     const SS_G uint16_t *deg16;
-    SS_G uint32_t *cnt;          // infectious-neighbour counts of this tick, null when this tick pulls
+    SS_G uint16_t *cnt;          // infectious-neighbour counts of this tick, null when this tick pulls
     int32_t last;
     int8_t s_code, inf_code, s_target;
     uint32_t seed_lo, seed_hi;
@@ -69,7 +69,7 @@ struct Cold {
     const uint8_t *tcur;         // tile tags of this tick, null when every tile runs
     const uint32_t *inf_T;
     int32_t *history;
-    uint32_t *pcnt;              // infectious-neighbour counts for the next tick
+    uint16_t *pcnt;              // infectious-neighbour counts for the next tick
     uint8_t *tnext;              // tile tags for the next tick
     int64_t n_hub, h64;          // agent -> tile mapping of the replica
     int32_t hub0, heavy0, light0;          // tag slots (see ss_rep_dev)
@@ -81,13 +81,13 @@ typedef SS_L Cold LCold;        // the record as it sits in LDS
 // Pushing to the next tick's counts: the fields, copied to registers once
 // per batch of rows.
 struct Pusher {
-    SS_G uint32_t *pcnt;
+    uint16_t *pcnt;
     SS_G uint8_t *tnext;
     int64_t n_hub, h64;
     int32_t hub0, heavy0, light0;
     uint8_t ptag;
     __device__ __forceinline__ explicit Pusher(const LCold *C)
-        : pcnt((SS_G uint32_t *)C->pcnt), tnext((SS_G uint8_t *)C->tnext), n_hub(C->n_hub), h64(C->h64),
+        : pcnt(C->pcnt), tnext((SS_G uint8_t *)C->tnext), n_hub(C->n_hub), h64(C->h64),
           hub0(C->hub0), heavy0(C->heavy0), light0(C->light0), ptag(C->ptag) {}
     // tag slot of agent v (see ss_rep_dev)
     __device__ __forceinline__ int64_t slot_of(int64_t v) const {
@@ -97,7 +97,7 @@ struct Pusher {
     }
     // one more infectious neighbour for agent c next tick; its tile runs then
     __device__ __forceinline__ void one(int32_t c) const {
-        __hip_atomic_fetch_add(pcnt + c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ss_count_add(pcnt, c);
         tnext[slot_of(c)] = ptag;
     }
     // entries [b, e) of col, by `width` threads from `me` on, four loads in
@@ -351,7 +351,7 @@ __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int
 #pragma unroll
         for (int a = 0; a < A; ++a) {
             const int64_t i = g0 + a * 64 + lane;
-            if (kc[a] != 0u) V.cnt[i] = 0u;           // ready for the tick after next
+            if (kc[a] != 0u) V.cnt[i] = 0;            // ready for the tick after next
             cand |= (s[a] == V.s_code && kc[a] != 0u ? 1u : 0u) << a;
         }
     } else {
@@ -558,9 +558,11 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
     if (nl > 0) {
         if (!pull) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint4 x = ld4(V.cnt + i0 + 4 * q);
-                a[4 * q] = x.x; a[4 * q + 1] = x.y; a[4 * q + 2] = x.z; a[4 * q + 3] = x.w;
+            for (int q = 0; q < 2; ++q) {
+                const uint4 x = ld4(V.cnt + i0 + 8 * q);
+                const uint32_t wd[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                for (int h = 0; h < 8; ++h) a[8 * q + h] = (wd[h >> 1] >> (16 * (h & 1))) & 0xFFFFu;
             }
         } else {
 #pragma unroll
@@ -578,9 +580,12 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
     if (!pull && nl > 0) {
         // counts are cleared as they are read, ready for the tick after next
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if ((a[4 * q] | a[4 * q + 1] | a[4 * q + 2] | a[4 * q + 3]) != 0u)
-                st4(V.cnt + i0 + 4 * q, make_uint4(0u, 0u, 0u, (uint32_t)opaque(0)));
+        for (int q = 0; q < 2; ++q) {
+            uint32_t any = 0u;
+#pragma unroll
+            for (int h = 0; h < 8; ++h) any |= a[8 * q + h];
+            if (any != 0u) st4(V.cnt + i0 + 8 * q, make_uint4(0u, 0u, 0u, (uint32_t)opaque(0)));
+        }
     }
     // states with a spontaneous transition, as a bit set over state codes 0..3
     uint32_t tbits = 0u;
@@ -793,7 +798,7 @@ __device__ __forceinline__ void heavy_tile(const TickView V, const LCold *C, int
             uint32_t kc = 1u;                        // pull: every susceptible agent is a candidate
             if (V.cnt) {
                 kc = V.cnt[i];
-                if (kc != 0u) V.cnt[i] = 0u;
+                if (kc != 0u) V.cnt[i] = 0;
             }
             if (s == V.s_code) {
                 ++w.s_agents;
@@ -858,7 +863,7 @@ __device__ __forceinline__ void hub_tile(const TickView V, const LCold *C, int64
         // one thread reads and clears the count; the block agrees on it
         if (tid == 0) {
             const uint32_t c = V.cnt[i];
-            if (c != 0u) V.cnt[i] = 0u;
+            if (c != 0u) V.cnt[i] = 0;
             *s_cnt = (int)c;
         }
         __syncthreads();
@@ -1089,7 +1094,7 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
                 // this tick writes the next tick's tags when the rule says so
                 const uint32_t dec = R->cnt[0] ? R->dec[t & 1] : 0u;
                 const bool produce = push_rule(R, t);
-                V.cnt = dec == 0u ? nullptr : (SS_G uint32_t *)R->cnt[t & 1];
+                V.cnt = dec == 0u ? nullptr : (SS_G uint16_t *)R->cnt[t & 1];
                 V.push_next = produce ? 1 : 0;
                 V.tag = ss_front_tag(t);
                 const uint8_t *tcur = dec == 1u ? R->tact[t & 1] : nullptr;
@@ -1154,19 +1159,19 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
 // walked by a wavefront each, the others by a thread.
 __global__ void count_init_kernel(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
                                   const uint64_t *__restrict__ bits, int64_t n_local, int64_t heavy,
-                                  uint32_t *cnt, const uint32_t *dec) {
+                                  uint16_t *cnt, const uint32_t *dec) {
     if (*dec == 0u) return;
     const int lane = threadIdx.x & 63;
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = gt >> 6; i < heavy; i += nt >> 6) {
         if (!((bits[i >> 6] >> (i & 63)) & 1ull)) continue;
         const int64_t b = row_ptr[i], e = row_ptr[i + 1];
-        for (int64_t j = b + lane; j < e; j += 64) atomicAdd(&cnt[col[j]], 1u);
+        for (int64_t j = b + lane; j < e; j += 64) ss_count_add(cnt, col[j]);
     }
     for (int64_t i = heavy + gt; i < n_local; i += nt) {
         if (!((bits[i >> 6] >> (i & 63)) & 1ull)) continue;
         const int64_t b = row_ptr[i], e = row_ptr[i + 1];
-        for (int64_t j = b; j < e; ++j) atomicAdd(&cnt[col[j]], 1u);
+        for (int64_t j = b; j < e; ++j) ss_count_add(cnt, col[j]);
     }
 }
 
@@ -1449,7 +1454,7 @@ hipError_t ss_launch_tick(const ss_rep_dev *reps, const ss_tile *tiles, int n_ti
 }
 
 hipError_t ss_launch_count_init(const int64_t *row_ptr, const int32_t *col_idx, const uint64_t *bits,
-                                int64_t n_local, int64_t heavy, uint32_t *cnt, uint32_t *dec,
+                                int64_t n_local, int64_t heavy, uint16_t *cnt, uint32_t *dec,
                                 const unsigned long long *acc_t, uint32_t push_num, int grid, hipStream_t s) {
     hipLaunchKernelGGL(dec_init_kernel, dim3(1), dim3(1), 0, s, acc_t, push_num, dec);
     if (n_local <= 0) return hipGetLastError();

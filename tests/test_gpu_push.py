@@ -190,3 +190,45 @@ def test_wide_tiles_shrunk_chunks(grid):
                 e.simulate(15)
                 assert first_diff(rep.states(), trace[t0 + 14]) is None, (rule, t0, first_diff(rep.states(), trace[t0 + 14]))
             rep.remove()
+
+
+@pytest.mark.parametrize("apl", [4, 16])
+def test_unsorted_rows(apl):
+    """Rows in random order (no degree sort): long rows sit among short ones in
+    light tiles and the tile sizes follow them; every rule gives the oracle's
+    states."""
+    g0 = graphs.barabasi_albert(60_000, 4, 71)
+    rng = np.random.default_rng(72)
+    p = rng.permutation(g0.n)
+    src = np.repeat(np.arange(g0.n), np.diff(g0.row_ptr))
+    g = graphs.from_edges(g0.n, p[src], p[g0.col_idx])      # every edge twice: mirrors stay
+    st = np.where(rng.random(g.n) < 0.002, 1, 0).astype(np.int8)
+    om = O.sir_model(0.05, 0.1, 73)
+    _, trace = O.sir_run(om, g.row_ptr, g.col_idx, np.arange(g.n), st, 60, trace=True)
+    say("unsorted rows", apl, int(np.diff(g.row_ptr).max()))
+    for rule in RULES:
+        with Engine(0) as e:
+            e.set_push(rule)
+            e.set_tiling(64, apl, 0, True)
+            rep = Replica(e, g, st, sir_model(0.05, 0.1), 73, sort=False)
+            for t0 in range(0, 60, 12):
+                e.simulate(12)
+                assert first_diff(rep.states(), trace[t0 + 11]) is None, (rule, t0, first_diff(rep.states(), trace[t0 + 11]))
+            rep.remove()
+    assert (trace[-1] == 2).sum() > g.n // 20
+
+
+def test_row_of_65535_or_more_stays_pull(engines):
+    """Counts are 16-bit: a replica with a row of 65535 entries or more never
+    pushes, and its states still equal the oracle's."""
+    n = 70_001
+    g = graphs.from_edges(n, np.zeros(n - 1, np.int64), np.arange(1, n, dtype=np.int64))
+    st = np.zeros(n, np.int8)
+    st[1:40] = 1
+    want, _ = O.sir_run(O.sir_model(0.02, 0.2, 81), g.row_ptr, g.col_idx, np.arange(n), st, 20)
+    e = engines[2]
+    rep = Replica(e, g, st, sir_model(0.02, 0.2), 81)
+    assert not e.push_ok(rep.rep)
+    e.simulate(20)
+    assert first_diff(rep.states(), want) is None
+    rep.remove()

@@ -77,13 +77,13 @@ def test_no_device_fails_cleanly():
 
 
 def test_footprint_counts_push_arrays_only_without_ghosts():
-    """A ghost-free replica holds two int32 count arrays for the push direction
-    (8 bytes per agent); a replica with ghost slots has none."""
+    """A ghost-free replica holds two 16-bit count arrays for the push direction
+    (4 bytes per agent); a replica with ghost slots has none."""
     L = _lib.step_lib()
     no_ghost = L.ss_replica_footprint(1000, 0, 10_000, 0, 50)
     with_ghost = L.ss_replica_footprint(1000, 1, 10_000, 0, 50)
     # one ghost slot adds 8 (row_ptr) + 3 (states) bytes and at most one bitmap word per buffer
-    assert no_ghost - with_ghost >= 8 * 1000 - 32
+    assert no_ghost - with_ghost >= 4 * 1000 - 32
 
 
 def test_footprint_is_positive_and_grows():
