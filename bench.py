@@ -11,7 +11,10 @@ Prints one JSON line: agent-ticks/s (whole job), the roofline of the tick
 kernel (state-aware algorithmic bytes of each tick over HIP-event launch
 time, beside the bytes the kernel's own counters say it touched) and the CPU
 oracle timed on this host's cores on windows of the same run (each window
-also checked bit-exact against the GPU).
+also checked bit-exact against the GPU).  Further legs, reported beside it:
+one SBM partition with ghosts and a host halo feed (hash-map ids, history),
+a batched ensemble of 256 scenarios, and a serving loop with continuous
+batching.
 """
 import argparse
 import json
@@ -88,6 +91,8 @@ def main():
     ap.add_argument("--ensemble-ticks", type=int, default=300)
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving leg: seconds of arrivals (0 skips)")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="serving leg: arrivals per second")
+    ap.add_argument("--partition-ticks", type=int, default=150,
+                    help="partition leg (one SBM part with ghosts and a halo feed): ticks (0 skips)")
     ap.add_argument("--plain", action="store_true", help="only the plain 10M-agent leg")
     args = ap.parse_args()
 
@@ -258,12 +263,70 @@ def main():
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
     rep.remove()
+    if not args.plain and args.partition_ticks > 0:
+        out["partition"] = partition_leg(eng, args)
     if not args.plain and args.ensemble_replicas > 0:
         out["ensemble"] = ensemble_leg(eng, args)
     if not args.plain and args.serve_seconds > 0:
         out["serving"] = serving_leg(eng, args)
     print(json.dumps(out), flush=True)
     eng.close()
+
+
+def partition_leg(eng, args):
+    """One partition's share of a 32M-agent stochastic block model (64 blocks,
+    mean degree 12 inside a block and 2 across, 8 parts, part 0): local rows,
+    ghost slots whose scrambled int64 ids are resolved through the GPU hash map
+    (load factor 0.5), every ghost's state refreshed from a seeded host halo
+    buffer before every tick, last-infector history on.  Timed in 10-tick
+    calls of ss_simulate_halo, halo upload included; the first 10 ticks are
+    checked against the oracle (states and history)."""
+    from sagesim_amd import Replica, sir_model, workloads
+    t0 = time.perf_counter()
+    pw = workloads.sbm_partition(ticks=args.partition_ticks)
+    g = pw.graph
+    n_ghost = g.n_cols - g.n
+    t_gen = time.perf_counter() - t0
+    eng.set_tiling()
+    t0 = time.perf_counter()
+    rep = Replica(eng, g, pw.state, sir_model(pw.beta, pw.gamma), pw.seed, rng_key_id=pw.local_ids,
+                  history=pw.history)
+    eng.map_build(rep.rep, pw.ghost_ids, np.arange(g.n, g.n_cols, dtype=np.int32), 0.5)
+    miss = eng.halo_bind(rep.rep, pw.halo_ids)
+    eng.sync()
+    t_up = time.perf_counter() - t0
+    log(f"partition: {g.n} local agents, {n_ghost} ghosts, {g.nnz} entries, built in {t_gen:.1f}s, "
+        f"uploaded in {t_up:.1f}s")
+    chunk = 10
+    t_run = 0.0
+    exact = None
+    for t in range(0, pw.ticks, chunk):
+        k = min(chunk, pw.ticks - t)
+        vals = pw.halo_vals(t, t + k)
+        eng.sync()
+        c0 = time.perf_counter()
+        eng.simulate_halo(rep.rep, k, vals)
+        eng.sync()
+        t_run += time.perf_counter() - c0
+        if t == 0 and not args.no_cpu:
+            from oracle import oracle as O
+            hist = pw.history.copy()
+            want, _ = O.sir_run(O.sir_model(pw.beta, pw.gamma, pw.seed), g.row_ptr, g.col_idx, pw.local_ids,
+                                pw.state, k, n_local=g.n, history=hist, ghost_ids=pw.ghost_ids,
+                                halo_ids=pw.halo_ids, halo_vals=vals, nthreads=host_threads())
+            exact = bool(np.array_equal(rep.states()[:g.n], want[:g.n]) and np.array_equal(rep.history(), hist))
+    final = np.bincount(rep.states()[:g.n], minlength=3).tolist()
+    rep.remove()
+    rate = g.n * pw.ticks / t_run
+    log(f"partition: {pw.ticks} ticks in {t_run * 1e3:.1f} ms ({rate:.3g} local agent-ticks/s), "
+        f"halo {n_ghost / 1e6:.2f} MB per tick, oracle window exact {exact}")
+    return {"workload": "SIR on SBM part 0 of 8 (64 blocks x 500k agents, mean degree 12 in, 2 out), "
+                        "ghost ids through the hash map, host halo every tick, history 32",
+            "local_agents": int(g.n), "ghosts": int(n_ghost), "nnz": int(g.nnz), "ticks": pw.ticks,
+            "halo_ids_missing": int(miss), "seconds": round(t_run, 4), "ms_per_tick": round(t_run * 1e3 / pw.ticks, 4),
+            "local_agent_ticks_per_s": rate, "halo_bytes_per_tick": int(n_ghost),
+            "oracle_window_ticks": chunk, "oracle_window_exact": exact, "final_SIR": final,
+            "graph_build_s": round(t_gen, 2), "upload_s": round(t_up, 2)}
 
 
 def ensemble_leg(eng, args):

@@ -75,9 +75,12 @@ class PartitionWorkload:
         rng = np.random.Generator(np.random.PCG64(self.seed + 99))
         t_inf = rng.integers(0, 2 * self.ticks, size=h)
         dur = rng.integers(1, 15, size=h)
-        t = np.arange(t0, t1)[:, None]
-        v = np.where(t < t_inf[None, :], 0, np.where(t < (t_inf + dur)[None, :], 1, 2))
-        return np.ascontiguousarray(v, np.int8)
+        t_rec = t_inf + dur
+        v = np.empty((max(0, t1 - t0), h), np.int8)
+        for k, t in enumerate(range(t0, t1)):
+            v[k] = t >= t_inf
+            v[k] += t >= t_rec
+        return v
 
 
 def sbm_partition(n_blocks=64, block_size=500_000, deg_in=12.0, deg_out=2.0, n_parts=8, part=0,
