@@ -300,9 +300,10 @@ def partition_leg(eng, args):
     chunk = 10
     t_run = 0.0
     exact = None
+    pinned = eng.pinned_empty((chunk, pw.halo_ids.shape[0]), np.int8)   # halo values need no staging
     for t in range(0, pw.ticks, chunk):
         k = min(chunk, pw.ticks - t)
-        vals = pw.halo_vals(t, t + k)
+        vals = pw.halo_vals(t, t + k, out=pinned[:k])
         eng.sync()
         c0 = time.perf_counter()
         eng.simulate_halo(rep.rep, k, vals)

@@ -151,9 +151,15 @@ int ss_halo_unpack_ids(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int8_
 /* out[h] = current state of slot slots[h] (the send side, gpu_kernels.py:852-909) */
 int ss_halo_pack(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n, int8_t *out);
 /* Run `ticks` ticks; before tick j the ghosts are refreshed from
- * vals[j * n_bound ...] (bound order), staged through pinned memory and
- * copied on a second stream while the previous tick computes. */
+ * vals[j * n_bound ...] (bound order), copied on a second stream while the
+ * previous tick computes: straight from vals when it lies in memory from
+ * ss_host_alloc, else staged through the library's pinned buffers. */
 int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals);
+/* Page-locked host memory for halo buffers (no counterpart in the reference:
+ * for callers that feed ghost states every tick, whose per-tick copy then
+ * needs no staging).  Freed with ss_host_free. */
+int ss_host_alloc(ss_ctx *ctx, int64_t bytes, void **out);
+int ss_host_free(ss_ctx *ctx, void *p);
 
 /* ---- step direction: no counterpart in the reference, whose generated
  *      kernel always reads every neighbour (model.py:2302-2408) ----

@@ -121,7 +121,7 @@ void free_replica(ss_replica &r) {
     dfree(r.history); dfree(r.init_history); dfree(r.inf_T);
     dfree(r.cnt[0]); dfree(r.cnt[1]); dfree(r.dec); dfree(r.acc);
     dfree(r.map_keys); dfree(r.map_vals); dfree(r.halo_slots);
-    dfree(r.halo_dev[0]); dfree(r.halo_dev[1]);
+    dfree(r.halo_dev[0]); dfree(r.halo_dev[1]); dfree(r.halo_rec);
     r = ss_replica();
 }
 
@@ -893,8 +893,8 @@ int ss_halo_bind(ss_ctx *ctx, int32_t rep, const int64_t *ids, int64_t n, int64_
     ss_replica &r = ctx->reps[rep];
     if (!r.map_keys) SS_FAIL(ctx, SS_ERR_STATE, "map not built");
     SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    dfree(r.halo_slots); dfree(r.halo_dev[0]); dfree(r.halo_dev[1]);
-    r.halo_slots = nullptr; r.halo_dev[0] = r.halo_dev[1] = nullptr;
+    dfree(r.halo_slots); dfree(r.halo_dev[0]); dfree(r.halo_dev[1]); dfree(r.halo_rec);
+    r.halo_slots = nullptr; r.halo_dev[0] = r.halo_dev[1] = nullptr; r.halo_rec = nullptr;
     r.n_halo = 0;
     if (n == 0) return SS_OK;
     SS_HIP(ctx, dalloc(ctx, &r, &r.halo_slots, (size_t)n));
@@ -903,6 +903,19 @@ int ss_halo_bind(ss_ctx *ctx, int32_t rep, const int64_t *ids, int64_t n, int64_
     int64_t miss = 0;
     int rc = lookup_dev(ctx, r, ids, r.halo_slots, n, &miss);
     if (rc) return rc;
+    // the refresh runs in slot order: record of every slot from the first
+    // bound one (or the first ghost) on; a slot bound twice takes the later record
+    std::vector<int32_t> slots((size_t)n);
+    SS_HIP(ctx, hipMemcpy(slots.data(), r.halo_slots, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost));
+    int64_t lo = r.n_local;
+    for (int32_t s : slots)
+        if (s >= 0) lo = std::min<int64_t>(lo, s);
+    r.halo_base = lo / 64 * 64;
+    std::vector<int32_t> rec((size_t)(r.n_total - r.halo_base), -1);
+    for (int64_t h = 0; h < n; ++h)
+        if (slots[(size_t)h] >= 0) rec[(size_t)(slots[(size_t)h] - r.halo_base)] = (int32_t)h;
+    SS_HIP(ctx, dalloc(ctx, &r, &r.halo_rec, rec.size()));
+    SS_HIP(ctx, hipMemcpy(r.halo_rec, rec.data(), sizeof(int32_t) * rec.size(), hipMemcpyHostToDevice));
     if (n_missing) *n_missing = miss;
     r.n_halo = n;
     return SS_OK;
@@ -920,8 +933,12 @@ int ss_halo_unpack(ss_ctx *ctx, int32_t rep, const int8_t *vals, int64_t n) {
     if (n == 0) return SS_OK;
     SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     SS_HIP(ctx, hipMemcpy(r.halo_dev[0], vals, (size_t)n, hipMemcpyHostToDevice));
-    SS_HIP(ctx, ss_launch_halo_scatter(r.state[0], r.state[1], r.halo_slots, r.halo_dev[0], n, ctx->stream));
-    return refresh_ghost_bits(ctx, r);
+    const int par = (int)(r.tick & 1);
+    SS_HIP(ctx, ss_launch_halo_gather(r.state[0], r.state[1], r.state[par], r.ibits[par], r.halo_rec, r.halo_dev[0],
+                                      r.halo_base, r.n_total, r.dev.inf_code, ctx->stream));
+    // the words below the refreshed range that hold ghosts
+    if (r.halo_base > r.n_local / 64 * 64) return rebuild_bits(ctx, r, par, r.n_local / 64, r.halo_base / 64);
+    return SS_OK;
 }
 
 int ss_halo_unpack_ids(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int8_t *vals, int64_t n, int64_t *n_missing) {
@@ -990,18 +1007,33 @@ int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals
         ctx->h_halo_cap = n;
     }
     if ((rc = push_descriptors(ctx, ticks))) return rc;
+    // values in page-locked memory are copied from where they are
+    bool pinned = false;
+    {
+        hipPointerAttribute_t at;
+        const hipError_t e = hipPointerGetAttributes(&at, vals);
+        pinned = e == hipSuccess && at.type == hipMemoryTypeHost;
+        if (e != hipSuccess) (void)hipGetLastError();   // pageable memory: not an error here
+    }
     const int64_t w0 = r.n_local / 64;
     for (int64_t k = 0; k < ticks; ++k) {
         const int b = (int)(k & 1);
-        // the pinned slot is free once its previous copy finished
-        SS_HIP(ctx, hipEventSynchronize(ctx->halo_ev[b]));
-        std::memcpy(ctx->h_halo_pin[b], vals + (size_t)k * (size_t)n, (size_t)n);
-        SS_HIP(ctx, hipMemcpyAsync(r.halo_dev[b], ctx->h_halo_pin[b], (size_t)n, hipMemcpyHostToDevice, ctx->copy_stream));
+        const int8_t *src = vals + (size_t)k * (size_t)n;
+        if (!pinned) {
+            // the pinned slot is free once its previous copy finished
+            SS_HIP(ctx, hipEventSynchronize(ctx->halo_ev[b]));
+            std::memcpy(ctx->h_halo_pin[b], src, (size_t)n);
+            src = ctx->h_halo_pin[b];
+        }
+        SS_HIP(ctx, hipMemcpyAsync(r.halo_dev[b], src, (size_t)n, hipMemcpyHostToDevice, ctx->copy_stream));
         SS_HIP(ctx, hipEventRecord(ctx->halo_ev[b], ctx->copy_stream));
         SS_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->halo_ev[b], 0));
-        SS_HIP(ctx, ss_launch_halo_scatter(r.state[0], r.state[1], r.halo_slots, r.halo_dev[b], n, ctx->stream));
         const int par = (int)((r.tick + k) & 1);
-        SS_HIP(ctx, ss_launch_bits(r.state[par], r.ibits[par], w0, r.n_words, r.n_total, r.dev.inf_code, ctx->stream));
+        SS_HIP(ctx, ss_launch_halo_gather(r.state[0], r.state[1], r.state[par], r.ibits[par], r.halo_rec,
+                                          r.halo_dev[b], r.halo_base, r.n_total, r.dev.inf_code, ctx->stream));
+        if (r.halo_base > w0 * 64)
+            SS_HIP(ctx, ss_launch_bits(r.state[par], r.ibits[par], w0, r.halo_base / 64, r.n_total, r.dev.inf_code,
+                                       ctx->stream));
         SS_HIP(ctx, tick_launch(ctx, k));
         // halo_dev[b] is rewritten two ticks later: the copy stream waits for this tick's scatter
         hipEvent_t done;
@@ -1010,7 +1042,23 @@ int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals
         SS_HIP(ctx, hipStreamWaitEvent(ctx->copy_stream, done, 0));
         (void)hipEventDestroy(done);
     }
+    // the caller's buffer is borrowed for this call only
+    if (pinned) SS_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));
     advance_ticks(ctx, ticks);
+    return SS_OK;
+}
+
+int ss_host_alloc(ss_ctx *ctx, int64_t bytes, void **out) {
+    if (!ctx || !out || bytes < 0) SS_FAIL(ctx, SS_ERR_INVALID, "bad argument");
+    *out = nullptr;
+    (void)hipSetDevice(ctx->device);
+    SS_HIP(ctx, hipHostMalloc(out, (size_t)std::max<int64_t>(bytes, 1), hipHostMallocDefault));
+    return SS_OK;
+}
+
+int ss_host_free(ss_ctx *ctx, void *p) {
+    if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
+    if (p) SS_HIP(ctx, hipHostFree(p));
     return SS_OK;
 }
 

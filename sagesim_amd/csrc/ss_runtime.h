@@ -34,8 +34,10 @@ hipError_t ss_launch_map_lookup(const unsigned long long *keys, const int32_t *v
                                 int32_t *out, int64_t n, int32_t *misses, hipStream_t s);
 hipError_t ss_launch_map_remove(unsigned long long *keys, int32_t *vals, int64_t cap, const int64_t *ids, int64_t n,
                                 int32_t *removed, hipStream_t s);
-hipError_t ss_launch_halo_scatter(int8_t *s0, int8_t *s1, const int32_t *slots, const int8_t *vals, int64_t n,
-                                  hipStream_t s);
+// ghost refresh from a bound halo: slot base + g <- vals[rec[g]] (rec -1: kept),
+// both buffers, and the bitmap words of [base, n_total) of the buffer `cur`
+hipError_t ss_launch_halo_gather(int8_t *s0, int8_t *s1, const int8_t *cur, uint64_t *bits, const int32_t *rec,
+                                 const int8_t *vals, int64_t base, int64_t n_total, int8_t inf, hipStream_t s);
 hipError_t ss_launch_halo_unpack_ids(const unsigned long long *keys, const int32_t *mvals, int64_t cap,
                                      const int64_t *ids, const int8_t *vals, int64_t n, int8_t *s0, int8_t *s1,
                                      int32_t *misses, hipStream_t s);
@@ -87,6 +89,8 @@ struct ss_replica {
     // bound halo: slot of every record of the per-tick halo buffer
     int32_t *halo_slots = nullptr;
     int64_t n_halo = 0;
+    int32_t *halo_rec = nullptr;                // record of slot halo_base + g, -1 when not bound
+    int64_t halo_base = 0;                      // first slot refreshed (a multiple of 64)
     int8_t *halo_dev[2] = {nullptr, nullptr};  // device staging of halo values
     // bookkeeping
     int64_t tick = 0, tick_init = 0;

@@ -1348,15 +1348,29 @@ __global__ void fill_i32_kernel(int32_t *p, int64_t n, int32_t v) {
 
 // Ghost refresh (gpu_kernels.py:1030-1036 writes ghosts into the read and the
 // write buffer): record h of the halo goes to slot slots[h].
-__global__ void halo_scatter_kernel(int8_t *s0, int8_t *s1, const int32_t *__restrict__ slots,
-                                    const int8_t *__restrict__ vals, int64_t n) {
+// Ghost refresh in slot order: slot base + g takes record rec[g] of the halo
+// (-1: not bound, state kept) into both state buffers, and the bitmap words
+// of the slots [base, n_total) are rebuilt from the states of the buffer the
+// next tick reads (`cur`, `bits`).  base is a multiple of 64.  One gather per
+// slot and whole-word writes, where a scatter in record order would write
+// single bytes all over the slots.
+__global__ void halo_gather_kernel(int8_t *s0, int8_t *s1, const int8_t *cur, uint64_t *__restrict__ bits,
+                                   const int32_t *__restrict__ rec, const int8_t *__restrict__ vals, int64_t base,
+                                   int64_t n_total, int8_t inf) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    const int32_t s = slots[g];
-    if (s < 0) return;
-    const int8_t v = vals[g];
-    s0[s] = v;
-    s1[s] = v;
+    const int64_t i = base + g;
+    const bool in = i < n_total;
+    const int32_t h = in ? rec[g] : -1;
+    int8_t v = 0;
+    if (h >= 0) {
+        v = vals[h];
+        s0[i] = v;
+        s1[i] = v;
+    } else if (in) {
+        v = cur[i];
+    }
+    const uint64_t word = __ballot(in && v == inf);
+    if ((threadIdx.x & 63) == 0 && in) bits[i >> 6] = word;
 }
 
 // Ghost refresh keyed by id: resolve through the map, then scatter.
@@ -1534,10 +1548,11 @@ hipError_t ss_launch_map_remove(unsigned long long *keys, int32_t *vals, int64_t
     return hipGetLastError();
 }
 
-hipError_t ss_launch_halo_scatter(int8_t *s0, int8_t *s1, const int32_t *slots, const int8_t *vals, int64_t n,
-                                  hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(halo_scatter_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, s0, s1, slots, vals, n);
+hipError_t ss_launch_halo_gather(int8_t *s0, int8_t *s1, const int8_t *cur, uint64_t *bits, const int32_t *rec,
+                                 const int8_t *vals, int64_t base, int64_t n_total, int8_t inf, hipStream_t s) {
+    if (n_total <= base) return hipSuccess;
+    hipLaunchKernelGGL(halo_gather_kernel, dim3(grid_for(n_total - base, 256)), dim3(256), 0, s, s0, s1, cur, bits,
+                       rec, vals, base, n_total, inf);
     return hipGetLastError();
 }
 

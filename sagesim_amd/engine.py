@@ -7,6 +7,7 @@ caller keeps its own agent order; the RNG stays keyed on the caller's ids, so
 results do not depend on the relabelling.
 """
 import ctypes
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -48,6 +49,7 @@ class Engine:
             raise NativeError(rc, self._L.ss_last_error(None).decode())
         self._h = h
         self.device = device
+        self._pinned = {}                  # address -> byte count of live page-locked buffers
 
     # -- plumbing --
     def _ck(self, rc):
@@ -56,8 +58,31 @@ class Engine:
 
     def close(self):
         if getattr(self, "_h", None):
+            for addr in list(self._pinned):
+                self._L.ss_host_free(self._h, ctypes.c_void_p(addr))
+            self._pinned.clear()
             self._L.ss_destroy(self._h)
             self._h = None
+
+    def pinned_empty(self, shape, dtype=np.int8):
+        """A numpy array in page-locked host memory (ss_host_alloc): halo values
+        passed to simulate_halo from such an array are copied with no staging.
+        The memory is released with the array, or when the engine closes."""
+        dt = np.dtype(dtype)
+        nbytes = int(np.prod(shape, dtype=np.int64)) * dt.itemsize
+        p = ctypes.c_void_p()
+        self._ck(self._L.ss_host_alloc(self._h, nbytes, ctypes.byref(p)))
+        addr = p.value
+        self._pinned[addr] = nbytes
+        buf = (ctypes.c_char * max(nbytes, 1)).from_address(addr)
+        arr = np.frombuffer(buf, dtype=dt, count=nbytes // dt.itemsize).reshape(shape)
+
+        def release(engine=weakref.ref(self), addr=addr):
+            e = engine()
+            if e is not None and e._h and e._pinned.pop(addr, None) is not None:
+                e._L.ss_host_free(e._h, ctypes.c_void_p(addr))
+        weakref.finalize(arr, release)
+        return arr
 
     def __del__(self):
         try:

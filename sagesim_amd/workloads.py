@@ -68,15 +68,17 @@ class PartitionWorkload:
     seed: int
     ticks: int
 
-    def halo_vals(self, t0, t1):
+    def halo_vals(self, t0, t1, out=None):
         """States carried by the halo for ticks [t0, t1): each ghost turns I at a
-        seeded tick and R a seeded number of ticks later."""
+        seeded tick and R a seeded number of ticks later.  Written into `out`
+        (shape (t1 - t0, n_halo), int8) when given."""
         h = self.halo_ids.shape[0]
         rng = np.random.Generator(np.random.PCG64(self.seed + 99))
         t_inf = rng.integers(0, 2 * self.ticks, size=h)
         dur = rng.integers(1, 15, size=h)
         t_rec = t_inf + dur
-        v = np.empty((max(0, t1 - t0), h), np.int8)
+        v = np.empty((max(0, t1 - t0), h), np.int8) if out is None else out
+        assert v.shape == (max(0, t1 - t0), h) and v.dtype == np.int8
         for k, t in enumerate(range(t0, t1)):
             v[k] = t >= t_inf
             v[k] += t >= t_rec

@@ -216,3 +216,35 @@ def test_bad_csr_rejected(eng):
     eng.simulate(2)
     assert rep.states().tolist() == [1, 1, 1]
     rep.remove()
+
+
+def test_halo_from_page_locked_buffer(eng):
+    """Halo values from a buffer of ss_host_alloc (copied with no staging) give
+    the same states and history as from an ordinary host array."""
+    pw = workloads.sbm_partition(n_blocks=8, block_size=3000, deg_in=10, deg_out=2, n_parts=2, part=0,
+                                 ticks=24, seed=19)
+    g = pw.graph
+    say("halo page-locked", g.n, g.n_cols)
+    out = []
+    for use_pinned in (False, True):
+        rep = Replica(eng, g, pw.state, sir_model(pw.beta, pw.gamma), pw.seed, rng_key_id=pw.local_ids,
+                      history=pw.history)
+        eng.map_build(rep.rep, pw.ghost_ids, np.arange(g.n, g.n_cols, dtype=np.int32), 0.5)
+        assert eng.halo_bind(rep.rep, pw.halo_ids) == 0
+        for t in range(0, pw.ticks, 8):
+            if use_pinned:
+                vals = eng.pinned_empty((8, pw.halo_ids.shape[0]), np.int8)
+                pw.halo_vals(t, t + 8, out=vals)
+            else:
+                vals = pw.halo_vals(t, t + 8)
+            eng.simulate_halo(rep.rep, 8, vals)
+        out.append((rep.states(), rep.history()))
+        rep.remove()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    hist = pw.history.copy()
+    want, _ = O.sir_run(O.sir_model(pw.beta, pw.gamma, pw.seed), g.row_ptr, g.col_idx, pw.local_ids, pw.state,
+                        pw.ticks, n_local=g.n, history=hist, ghost_ids=pw.ghost_ids, halo_ids=pw.halo_ids,
+                        halo_vals=pw.halo_vals(0, pw.ticks))
+    assert first_diff(out[1][0][:g.n], want[:g.n]) is None
+    assert np.array_equal(out[1][1], hist)
