@@ -390,6 +390,8 @@ int ss_create(int device, ss_ctx **out) {
         (e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&ctx->halo_ev[0], hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&ctx->halo_ev[1], hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&ctx->halo_used[0], hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&ctx->halo_used[1], hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreate(&ctx->t_start)) != hipSuccess || (e = hipEventCreate(&ctx->t_stop)) != hipSuccess ||
         (e = hipMalloc((void **)&ctx->d_work, SS_WORK_N * sizeof(unsigned long long))) != hipSuccess ||
         (e = hipMalloc((void **)&ctx->d_stats, 8 * sizeof(unsigned long long))) != hipSuccess ||
@@ -424,6 +426,7 @@ int ss_destroy(ss_ctx *ctx) {
     for (int i = 0; i < 2; ++i) {
         if (ctx->h_halo_pin[i]) (void)hipHostFree(ctx->h_halo_pin[i]);
         if (ctx->halo_ev[i]) (void)hipEventDestroy(ctx->halo_ev[i]);
+        if (ctx->halo_used[i]) (void)hipEventDestroy(ctx->halo_used[i]);
     }
     (void)hipStreamDestroy(ctx->stream);
     (void)hipStreamDestroy(ctx->copy_stream);
@@ -1025,22 +1028,20 @@ int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals
             std::memcpy(ctx->h_halo_pin[b], src, (size_t)n);
             src = ctx->h_halo_pin[b];
         }
+        // device slot b is free once the refresh that last read it (two ticks
+        // back) has run; the copy then overlaps the previous tick's kernels
+        SS_HIP(ctx, hipStreamWaitEvent(ctx->copy_stream, ctx->halo_used[b], 0));
         SS_HIP(ctx, hipMemcpyAsync(r.halo_dev[b], src, (size_t)n, hipMemcpyHostToDevice, ctx->copy_stream));
         SS_HIP(ctx, hipEventRecord(ctx->halo_ev[b], ctx->copy_stream));
         SS_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->halo_ev[b], 0));
         const int par = (int)((r.tick + k) & 1);
         SS_HIP(ctx, ss_launch_halo_gather(r.state[0], r.state[1], r.state[par], r.ibits[par], r.halo_rec,
                                           r.halo_dev[b], r.halo_base, r.n_total, r.dev.inf_code, ctx->stream));
+        SS_HIP(ctx, hipEventRecord(ctx->halo_used[b], ctx->stream));
         if (r.halo_base > w0 * 64)
             SS_HIP(ctx, ss_launch_bits(r.state[par], r.ibits[par], w0, r.halo_base / 64, r.n_total, r.dev.inf_code,
                                        ctx->stream));
         SS_HIP(ctx, tick_launch(ctx, k));
-        // halo_dev[b] is rewritten two ticks later: the copy stream waits for this tick's scatter
-        hipEvent_t done;
-        SS_HIP(ctx, hipEventCreateWithFlags(&done, hipEventDisableTiming));
-        SS_HIP(ctx, hipEventRecord(done, ctx->stream));
-        SS_HIP(ctx, hipStreamWaitEvent(ctx->copy_stream, done, 0));
-        (void)hipEventDestroy(done);
     }
     // the caller's buffer is borrowed for this call only
     if (pinned) SS_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));

@@ -135,7 +135,8 @@ struct ss_ctx {
     hipEvent_t t_start = nullptr, t_stop = nullptr;
     int8_t *h_halo_pin[2] = {nullptr, nullptr}; // pinned staging for per-tick halo values
     int64_t h_halo_cap = 0;
-    hipEvent_t halo_ev[2] = {nullptr, nullptr};
+    hipEvent_t halo_ev[2] = {nullptr, nullptr};   // halo slot b copied to the device
+    hipEvent_t halo_used[2] = {nullptr, nullptr}; // the tick that read device halo slot b is done
     std::string err;
     size_t bytes = 0;
 };
