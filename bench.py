@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--m", type=int, default=5)
     ap.add_argument("--ticks", type=int, default=200)
     ap.add_argument("--hub", type=int, default=1024, help="degree from which a row gets a whole block")
-    ap.add_argument("--heavy", type=int, default=128, help="degree from which a row gets a whole wavefront")
+    ap.add_argument("--heavy", type=int, default=256, help="degree from which a row gets a whole wavefront")
     ap.add_argument("--apl", type=int, default=16, help="agents per lane of the other rows")
     ap.add_argument("--no-autotune", action="store_true", help="use --hub/--heavy/--apl as given")
     ap.add_argument("--copy-mib", type=int, default=1024,

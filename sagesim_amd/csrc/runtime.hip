@@ -169,9 +169,9 @@ void drop_graphs(ss_ctx *ctx) {
 // start 64-aligned: they write whole bitmap words, the hub and heavy tiles set
 // single bits).  A light tile holds 256 * apl agents; with 16 agents per lane
 // it holds 4 * 1024, fewer where its rows are longer than the replica's
-// average light row: a tile of rows of degree d > dbar holds (dbar / d)^1.5
-// of that, in steps of 256 agents, so that the tiles at the long-row end cost
-// about as much as the others.  Blocks stride over the list (see below).
+// average light row: a tile of rows of degree d > dbar holds dbar / d of that
+// (about as many CSR entries as an average tile), in steps of 256 agents, so
+// that the tiles at the long-row end cost about as much as the others.  Blocks stride over the list (see below).
 int build_tiles(ss_ctx *ctx) {
     std::vector<ss_tile> hubs, heavy, light;
     int bpc = 0;
@@ -203,7 +203,7 @@ int build_tiles(ss_ctx *ctx) {
             const int64_t g1 = std::min<int64_t>(b + 64, nt);
             const double d = (double)(E(g1) - E(b)) / (double)(g1 - b);
             int64_t a = cap;
-            if (ctx->apl == 16 && dbar > 0.0 && d > dbar) a = (int64_t)((double)cap * std::pow(dbar / d, 1.5));
+            if (ctx->apl == 16 && dbar > 0.0 && d > dbar) a = (int64_t)((double)cap * dbar / d);
             // whole groups of 4 x 64 agents: the 4 wavefronts of a wide tile get equal chunks
             const int64_t step = ctx->apl == 16 ? 256 : 64;
             a = std::max<int64_t>(step, std::min(a, cap) / step * step);

@@ -121,7 +121,7 @@ struct ss_ctx {
     bool tiles_dirty = true;
     // tiling (autotuned by the host library)
     int hub_deg = 1024;
-    int heavy_deg = 128;
+    int heavy_deg = 256;
     int apl = 16;
     int grid_blocks = 0;                        // 0: persistent grid from occupancy
     int use_graph = 1;

@@ -168,7 +168,7 @@ class Engine:
         return out
 
     # -- run --
-    def set_tiling(self, heavy_degree=128, agents_per_lane=16, grid_blocks=0, use_graph=True, hub_degree=1024):
+    def set_tiling(self, heavy_degree=256, agents_per_lane=16, grid_blocks=0, use_graph=True, hub_degree=1024):
         """Launch shape: rows of degree >= hub_degree get a block, >= heavy_degree a
         wavefront, the rest agents_per_lane agents per lane (hub_degree is raised
         to heavy_degree when below it)."""
