@@ -1207,9 +1207,23 @@ __global__ void acc_init_kernel(const int8_t *__restrict__ st, const int64_t *__
         d += __shfl_down(d, off);
         c += __shfl_down(c, off);
     }
+    // one pair of atomics per block, spread over the slots (same-address
+    // atomics serialise)
+    __shared__ unsigned long long red[2 * 256 / 64];
+    const int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        if (d) atomicAdd(&acc[0], d);
-        if (c) atomicAdd(&acc[1], c);
+        red[2 * wv] = d;
+        red[2 * wv + 1] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+            d += red[2 * q];
+            c += red[2 * q + 1];
+        }
+        const int slot = (int)(blockIdx.x % SS_ACC_SLOTS);
+        if (d) atomicAdd(&acc[2 * slot], d);
+        if (c) atomicAdd(&acc[2 * slot + 1], c);
     }
 }
 
