@@ -189,7 +189,7 @@ int build_tiles(ss_ctx *ctx) {
         if (H < 0 || Hh < 0) SS_FAIL(ctx, SS_ERR_INVALID, "degree threshold without a prefix table");
         const int64_t nt = r.n_total;
         const int64_t H64 = std::min<int64_t>((H + 63) / 64 * 64, nt);
-        // hub count rounded up to 8: the 8 agents of a heavy wavefront then
+        // hub count rounded up to 8: the agents of a heavy wavefront (4) then
         // always share one bitmap word
         const int64_t nh = std::min((Hh + 7) / 8 * 8, H64);
         pos[ri] = Pos{(int64_t)hubs.size(), (int64_t)heavy.size(), (int64_t)light.size(), n_lblocks};

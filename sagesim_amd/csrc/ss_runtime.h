@@ -17,9 +17,9 @@
 // kernel launchers (tick_kernels.hip)
 hipError_t ss_launch_tick(const ss_rep_dev *reps, const ss_tile *tiles, int n_tiles, int k, int apl, int grid,
                           unsigned long long *counters, hipStream_t st);
-// agents of one HEAVY tile (4 wavefronts x 8 agents, tick_kernels.hip SS_HPW)
-#define SS_HEAVY_TILE 32
-#define SS_HEAVY_SHIFT 5
+// agents of one HEAVY tile (4 wavefronts x 4 agents, tick_kernels.hip SS_HPW)
+#define SS_HEAVY_TILE 16
+#define SS_HEAVY_SHIFT 4
 // degree thresholds that have a precomputed prefix length per replica
 #define SS_N_THRESHOLDS 16
 static const int32_t SS_THRESHOLDS[SS_N_THRESHOLDS] = {8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 512,

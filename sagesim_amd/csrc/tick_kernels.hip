@@ -779,7 +779,7 @@ __device__ __forceinline__ void put_bits(SS_G uint64_t *nb, int64_t word, uint64
 // HEAVY tile: SS_HPW agents per wavefront.  Lanes 0..SS_HPW-1 load, draw and
 // decide for their agent in parallel; the rows that need a scan are then
 // scanned one after another by the whole wavefront.
-#define SS_HPW 8
+#define SS_HPW 4
 __device__ __forceinline__ void heavy_tile(const TickView V, const LCold *C, int64_t begin, int64_t end,
                                            int64_t tix, Work &w, Dir &dir) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
