@@ -487,7 +487,7 @@ __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int
 // whole wavefront.  Results return through a byte per chunk slot.
 #define SS_WIDE_A 16
 #define SS_WIDE_CHUNK (64 * SS_WIDE_A)
-#define SS_WIDE_SHORT 64         // rows longer than this are scanned by the whole wavefront
+#define SS_WIDE_SHORT 128        // rows longer than this are scanned by the whole wavefront
 struct alignas(16) WideQueue {
     // draw entry: x = RNG key (pull) or infectious-neighbour count (push);
     //   y = slot | transition << 10 | (state & 3) << 11 | deg16 << 16 (deg16 only on pull)
