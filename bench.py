@@ -12,9 +12,9 @@ kernel (state-aware algorithmic bytes of each tick over HIP-event launch
 time, beside the bytes the kernel's own counters say it touched) and the CPU
 oracle timed on this host's cores on windows of the same run (each window
 also checked bit-exact against the GPU).  Further legs, reported beside it:
-one SBM partition with ghosts and a host halo feed (hash-map ids, history),
-a batched ensemble of 256 scenarios, and a serving loop with continuous
-batching.
+the reference's SIR step function compiled by hipRTC, one SBM partition with
+ghosts and a host halo feed (hash-map ids, history), a batched ensemble of
+256 scenarios, and a serving loop with continuous batching.
 """
 import argparse
 import json
@@ -93,6 +93,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=50.0, help="serving leg: arrivals per second")
     ap.add_argument("--partition-ticks", type=int, default=150,
                     help="partition leg (one SBM part with ghosts and a halo feed): ticks (0 skips)")
+    ap.add_argument("--stepfunc-n", type=int, default=1_000_000,
+                    help="step-function leg (the reference's SIR step compiled by hipRTC): agents (0 skips)")
+    ap.add_argument("--stepfunc-ticks", type=int, default=100)
     ap.add_argument("--plain", action="store_true", help="only the plain 10M-agent leg")
     args = ap.parse_args()
 
@@ -265,6 +268,8 @@ def main():
     rep.remove()
     if not args.plain and args.partition_ticks > 0:
         out["partition"] = partition_leg(eng, args)
+    if not args.plain and args.stepfunc_n > 0:
+        out["stepfunc"] = stepfunc_leg(args)
     if not args.plain and args.ensemble_replicas > 0:
         out["ensemble"] = ensemble_leg(eng, args)
     if not args.plain and args.serve_seconds > 0:
@@ -328,6 +333,60 @@ def partition_leg(eng, args):
             "local_agent_ticks_per_s": rate, "halo_bytes_per_tick": int(n_ghost),
             "oracle_window_ticks": chunk, "oracle_window_exact": exact, "final_SIR": final,
             "graph_build_s": round(t_gen, 2), "upload_s": round(t_up, 2)}
+
+
+def stepfunc_leg(args):
+    """The general path: the reference's SIR step function (HIP source with
+    the reference's argument order, float32 states 1/2/3, the reference's
+    Philox-2x32 uniforms) registered through Breed / Model / NetworkSpace,
+    compiled by hipRTC, on BA(n, 5).  The first 10 ticks are checked against
+    the oracle's restatement of the reference step."""
+    from sagesim_amd import Breed, Model, NetworkSpace, graphs
+    from sagesim_amd.steps import reference_sir_step
+    n, ticks, seed = args.stepfunc_n, args.stepfunc_ticks, 2024
+    t0 = time.perf_counter()
+    g = graphs.barabasi_albert(n, 5, 17)
+
+    class SIRBreed(Breed):
+        def __init__(self):
+            super().__init__("SIR")
+            self.register_property("state", 1)
+            self.register_step_func(reference_sir_step(), priority=0)
+
+    m = Model(NetworkSpace())
+    breed = SIRBreed()
+    m.register_breed(breed)
+    m.register_global_property("p_infection", 0.03)
+    m.register_global_property("p_recovery", 0.1)
+    m.set_seed(seed)
+    rng = np.random.default_rng(5)
+    init = np.where(rng.random(n) < 0.001, 2, 1)
+    for i in range(n):
+        m.create_agent_of_breed(breed, state=int(init[i]))
+    src = np.repeat(np.arange(n, dtype=np.int64), np.diff(g.row_ptr))
+    m.get_space().connect_agents_bulk(src, g.col_idx.astype(np.int64), directed=True)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    m.setup()
+    t_setup = time.perf_counter() - t0
+    exact = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        m.simulate(10)
+        want = O.ref_sir_run(g.row_ptr, g.col_idx, np.arange(n, dtype=np.float32), init.astype(np.float32), seed,
+                             10, 0.03, 0.1)
+        got = np.asarray(m._device_property("state"), np.float32).reshape(-1)[m._inv]
+        exact = bool(np.array_equal(got, want))
+    c0 = time.perf_counter()
+    m.simulate(ticks)
+    dt = time.perf_counter() - c0
+    rate = n * ticks / dt
+    log(f"stepfunc: BA({n}, 5), {ticks} ticks of the hipRTC-compiled reference step in {dt * 1e3:.1f} ms "
+        f"({rate:.3g} agent-ticks/s), mode {m.mode}, oracle window exact {exact}")
+    return {"workload": f"reference SIR step function (hipRTC), BA({n}, 5), float32 states, Philox-2x32",
+            "agents": n, "nnz": int(g.nnz), "ticks": ticks, "seconds": round(dt, 4), "agent_ticks_per_s": rate,
+            "mode": m.mode, "oracle_window_ticks": 10, "oracle_window_exact": exact,
+            "build_s": round(t_build, 2), "setup_s": round(t_setup, 2)}
 
 
 def ensemble_leg(eng, args):
