@@ -155,6 +155,16 @@ int ss_halo_pack(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n, int8
  * previous tick computes: straight from vals when it lies in memory from
  * ss_host_alloc, else staged through the library's pinned buffers. */
 int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals);
+/* 2-bit halo (no counterpart in the reference, whose halo carries one f32
+ * per property and agent, gpu_kernels.py:852-909): the same three calls with
+ * state codes packed four records a byte, record h in bits 2*(h%4) and
+ * 2*(h%4)+1 of byte h/4, (n + 3) / 4 bytes per halo of n records: a
+ * quarter of the bytes over the host link.  Every model has at most 4
+ * states (ss_set_model), so its codes fit; the pack side takes states
+ * modulo 4. */
+int ss_halo_unpack_2bit(ss_ctx *ctx, int32_t rep, const uint8_t *packed, int64_t n);
+int ss_halo_pack_2bit(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n, uint8_t *out);
+int ss_simulate_halo_2bit(ss_ctx *ctx, int32_t rep, int64_t ticks, const uint8_t *packed);
 /* Page-locked host memory for halo buffers (no counterpart in the reference:
  * for callers that feed ghost states every tick, whose per-tick copy then
  * needs no staging).  Freed with ss_host_free. */

@@ -8,7 +8,7 @@ The reference's API (Model, Breed, NetworkSpace) runs user step functions
 written as HIP device source (compiled with hipRTC) or, for the built-in
 compartment steps, the hand-written fused tick kernel.
 """
-from .engine import Engine, Replica, sir_model, seir_model  # noqa: F401
+from .engine import Engine, Replica, sir_model, seir_model, pack_2bit, unpack_2bit  # noqa: F401
 from . import graphs  # noqa: F401
 from .breed import Breed, StepFunction  # noqa: F401
 from .space import NetworkSpace  # noqa: F401

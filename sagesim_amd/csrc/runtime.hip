@@ -929,19 +929,27 @@ static int refresh_ghost_bits(ss_ctx *ctx, ss_replica &r) {
     return rebuild_bits(ctx, r, par, r.n_local / 64, r.n_words);
 }
 
-int ss_halo_unpack(ss_ctx *ctx, int32_t rep, const int8_t *vals, int64_t n) {
+static int halo_unpack(ss_ctx *ctx, int32_t rep, const void *vals, int64_t n, bool packed) {
     if (!valid_rep(ctx, rep) || (n && !vals)) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
     ss_replica &r = ctx->reps[rep];
     if (n != r.n_halo) SS_FAIL(ctx, SS_ERR_INVALID, "n = %lld, bound halo has %lld records", (long long)n, (long long)r.n_halo);
     if (n == 0) return SS_OK;
     SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    SS_HIP(ctx, hipMemcpy(r.halo_dev[0], vals, (size_t)n, hipMemcpyHostToDevice));
+    SS_HIP(ctx, hipMemcpy(r.halo_dev[0], vals, (size_t)(packed ? (n + 3) / 4 : n), hipMemcpyHostToDevice));
     const int par = (int)(r.tick & 1);
     SS_HIP(ctx, ss_launch_halo_gather(r.state[0], r.state[1], r.state[par], r.ibits[par], r.halo_rec, r.halo_dev[0],
-                                      r.halo_base, r.n_total, r.dev.inf_code, ctx->stream));
+                                      r.halo_base, r.n_total, r.dev.inf_code, packed, ctx->stream));
     // the words below the refreshed range that hold ghosts
     if (r.halo_base > r.n_local / 64 * 64) return rebuild_bits(ctx, r, par, r.n_local / 64, r.halo_base / 64);
     return SS_OK;
+}
+
+int ss_halo_unpack(ss_ctx *ctx, int32_t rep, const int8_t *vals, int64_t n) {
+    return halo_unpack(ctx, rep, vals, n, false);
+}
+
+int ss_halo_unpack_2bit(ss_ctx *ctx, int32_t rep, const uint8_t *packed, int64_t n) {
+    return halo_unpack(ctx, rep, packed, n, true);
 }
 
 int ss_halo_unpack_ids(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int8_t *vals, int64_t n, int64_t *n_missing) {
@@ -969,33 +977,45 @@ int ss_halo_unpack_ids(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int8_
     return refresh_ghost_bits(ctx, r);
 }
 
-int ss_halo_pack(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n, int8_t *out) {
+static int halo_pack(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n, void *out, bool packed) {
     if (!valid_rep(ctx, rep) || (n && (!slots || !out))) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
     ss_replica &r = ctx->reps[rep];
     for (int64_t i = 0; i < n; ++i)
         if (slots[i] < 0 || slots[i] >= r.n_total) SS_FAIL(ctx, SS_ERR_RANGE, "slot %d outside the replica", slots[i]);
     if (n == 0) return SS_OK;
+    const size_t nb = (size_t)(packed ? (n + 3) / 4 : n);
     SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     int32_t *d_slots = nullptr;
     int8_t *d_out = nullptr;
     SS_HIP(ctx, hipMalloc((void **)&d_slots, sizeof(int32_t) * n));
-    hipError_t e = hipMalloc((void **)&d_out, (size_t)n);
+    hipError_t e = hipMalloc((void **)&d_out, nb);
     if (e != hipSuccess) { dfree(d_slots); SS_HIP(ctx, e); }
     e = hipMemcpy(d_slots, slots, sizeof(int32_t) * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = ss_launch_gather(r.state[r.tick & 1], d_slots, d_out, n, ctx->stream);
+    if (e == hipSuccess)
+        e = packed ? ss_launch_gather2(r.state[r.tick & 1], d_slots, (uint8_t *)d_out, n, ctx->stream)
+                   : ss_launch_gather(r.state[r.tick & 1], d_slots, d_out, n, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e == hipSuccess) e = hipMemcpy(out, d_out, (size_t)n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out, d_out, nb, hipMemcpyDeviceToHost);
     dfree(d_slots);
     dfree(d_out);
     SS_HIP(ctx, e);
     return SS_OK;
 }
 
-int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals) {
+int ss_halo_pack(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n, int8_t *out) {
+    return halo_pack(ctx, rep, slots, n, out, false);
+}
+
+int ss_halo_pack_2bit(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n, uint8_t *out) {
+    return halo_pack(ctx, rep, slots, n, out, true);
+}
+
+static int simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals, bool packed) {
     if (!valid_rep(ctx, rep) || (ticks > 0 && !vals)) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
     if (ticks < 0 || ticks > INT32_MAX) SS_FAIL(ctx, SS_ERR_INVALID, "ticks out of range");
     ss_replica &r = ctx->reps[rep];
-    const int64_t n = r.n_halo;
+    // bytes of one tick's halo
+    const int64_t n = packed ? (r.n_halo + 3) / 4 : r.n_halo;
     if (n == 0) return ss_simulate(ctx, ticks);
     int rc;
     if (ctx->tiles_dirty && (rc = build_tiles(ctx))) return rc;
@@ -1036,7 +1056,8 @@ int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals
         SS_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->halo_ev[b], 0));
         const int par = (int)((r.tick + k) & 1);
         SS_HIP(ctx, ss_launch_halo_gather(r.state[0], r.state[1], r.state[par], r.ibits[par], r.halo_rec,
-                                          r.halo_dev[b], r.halo_base, r.n_total, r.dev.inf_code, ctx->stream));
+                                          r.halo_dev[b], r.halo_base, r.n_total, r.dev.inf_code, packed,
+                                          ctx->stream));
         SS_HIP(ctx, hipEventRecord(ctx->halo_used[b], ctx->stream));
         if (r.halo_base > w0 * 64)
             SS_HIP(ctx, ss_launch_bits(r.state[par], r.ibits[par], w0, r.halo_base / 64, r.n_total, r.dev.inf_code,
@@ -1047,6 +1068,14 @@ int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals
     if (pinned) SS_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));
     advance_ticks(ctx, ticks);
     return SS_OK;
+}
+
+int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals) {
+    return simulate_halo(ctx, rep, ticks, vals, false);
+}
+
+int ss_simulate_halo_2bit(ss_ctx *ctx, int32_t rep, int64_t ticks, const uint8_t *packed) {
+    return simulate_halo(ctx, rep, ticks, (const int8_t *)packed, true);
 }
 
 int ss_host_alloc(ss_ctx *ctx, int64_t bytes, void **out) {

@@ -35,13 +35,17 @@ hipError_t ss_launch_map_lookup(const unsigned long long *keys, const int32_t *v
 hipError_t ss_launch_map_remove(unsigned long long *keys, int32_t *vals, int64_t cap, const int64_t *ids, int64_t n,
                                 int32_t *removed, hipStream_t s);
 // ghost refresh from a bound halo: slot base + g <- vals[rec[g]] (rec -1: kept),
-// both buffers, and the bitmap words of [base, n_total) of the buffer `cur`
+// both buffers, and the bitmap words of [base, n_total) of the buffer `cur`;
+// packed: vals holds 2-bit codes, four records a byte
 hipError_t ss_launch_halo_gather(int8_t *s0, int8_t *s1, const int8_t *cur, uint64_t *bits, const int32_t *rec,
-                                 const int8_t *vals, int64_t base, int64_t n_total, int8_t inf, hipStream_t s);
+                                 const int8_t *vals, int64_t base, int64_t n_total, int8_t inf, bool packed,
+                                 hipStream_t s);
 hipError_t ss_launch_halo_unpack_ids(const unsigned long long *keys, const int32_t *mvals, int64_t cap,
                                      const int64_t *ids, const int8_t *vals, int64_t n, int8_t *s0, int8_t *s1,
                                      int32_t *misses, hipStream_t s);
 hipError_t ss_launch_gather(const int8_t *st, const int32_t *slots, int8_t *out, int64_t n, hipStream_t s);
+// out[h / 4] bits 2 (h % 4).. = st[slots[h]] & 3
+hipError_t ss_launch_gather2(const int8_t *st, const int32_t *slots, uint8_t *out, int64_t n, hipStream_t s);
 hipError_t ss_launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s);
 hipError_t ss_launch_copy(const void *src, void *dst, int64_t bytes, int grid, hipStream_t s);
 hipError_t ss_launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, int grid, hipStream_t s);

@@ -1368,6 +1368,9 @@ __global__ void fill_i32_kernel(int32_t *p, int64_t n, int32_t v) {
 // next tick reads (`cur`, `bits`).  base is a multiple of 64.  One gather per
 // slot and whole-word writes, where a scatter in record order would write
 // single bytes all over the slots.
+// PACKED: the halo holds 2-bit state codes, record h in bits 2 (h % 4) and
+// 2 (h % 4) + 1 of byte h / 4.
+template <bool PACKED>
 __global__ void halo_gather_kernel(int8_t *s0, int8_t *s1, const int8_t *cur, uint64_t *__restrict__ bits,
                                    const int32_t *__restrict__ rec, const int8_t *__restrict__ vals, int64_t base,
                                    int64_t n_total, int8_t inf) {
@@ -1377,7 +1380,10 @@ __global__ void halo_gather_kernel(int8_t *s0, int8_t *s1, const int8_t *cur, ui
     const int32_t h = in ? rec[g] : -1;
     int8_t v = 0;
     if (h >= 0) {
-        v = vals[h];
+        if constexpr (PACKED)
+            v = (int8_t)(((uint32_t)(uint8_t)vals[h >> 2] >> ((h & 3) * 2)) & 3u);
+        else
+            v = vals[h];
         s0[i] = v;
         s1[i] = v;
     } else if (in) {
@@ -1405,6 +1411,19 @@ __global__ void gather_kernel(const int8_t *__restrict__ st, const int32_t *__re
                               int8_t *__restrict__ out, int64_t n) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n) out[g] = st[slots[g]];
+}
+
+// The same as 2-bit codes, four records a byte (record h in bits 2 (h % 4)
+// and 2 (h % 4) + 1 of byte h / 4); states are taken modulo 4.
+__global__ void gather2_kernel(const int8_t *__restrict__ st, const int32_t *__restrict__ slots,
+                               uint8_t *__restrict__ out, int64_t n) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (4 * j >= n) return;
+    uint32_t b = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (4 * j + q < n) b |= ((uint32_t)(uint8_t)st[slots[4 * j + q]] & 3u) << (2 * q);
+    out[j] = (uint8_t)b;
 }
 
 // Census of one state buffer: out[s] += #agents in state s (s < 4) over the
@@ -1563,10 +1582,15 @@ hipError_t ss_launch_map_remove(unsigned long long *keys, int32_t *vals, int64_t
 }
 
 hipError_t ss_launch_halo_gather(int8_t *s0, int8_t *s1, const int8_t *cur, uint64_t *bits, const int32_t *rec,
-                                 const int8_t *vals, int64_t base, int64_t n_total, int8_t inf, hipStream_t s) {
+                                 const int8_t *vals, int64_t base, int64_t n_total, int8_t inf, bool packed,
+                                 hipStream_t s) {
     if (n_total <= base) return hipSuccess;
-    hipLaunchKernelGGL(halo_gather_kernel, dim3(grid_for(n_total - base, 256)), dim3(256), 0, s, s0, s1, cur, bits,
-                       rec, vals, base, n_total, inf);
+    if (packed)
+        hipLaunchKernelGGL(halo_gather_kernel<true>, dim3(grid_for(n_total - base, 256)), dim3(256), 0, s, s0, s1,
+                           cur, bits, rec, vals, base, n_total, inf);
+    else
+        hipLaunchKernelGGL(halo_gather_kernel<false>, dim3(grid_for(n_total - base, 256)), dim3(256), 0, s, s0, s1,
+                           cur, bits, rec, vals, base, n_total, inf);
     return hipGetLastError();
 }
 
@@ -1582,6 +1606,12 @@ hipError_t ss_launch_halo_unpack_ids(const unsigned long long *keys, const int32
 hipError_t ss_launch_gather(const int8_t *st, const int32_t *slots, int8_t *out, int64_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(gather_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, st, slots, out, n);
+    return hipGetLastError();
+}
+
+hipError_t ss_launch_gather2(const int8_t *st, const int32_t *slots, uint8_t *out, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather2_kernel, dim3(grid_for((n + 3) / 4, 256)), dim3(256), 0, s, st, slots, out, n);
     return hipGetLastError();
 }
 

@@ -50,6 +50,7 @@ class Engine:
         self._h = h
         self.device = device
         self._pinned = {}                  # address -> byte count of live page-locked buffers
+        self._halo_n = {}                  # replica -> records of its bound halo
 
     # -- plumbing --
     def _ck(self, rc):
@@ -130,6 +131,7 @@ class Engine:
 
     def remove_replica(self, rep):
         self._ck(self._L.ss_replica_destroy(self._h, rep))
+        self._halo_n.pop(rep, None)
 
     def reset(self, rep):
         self._ck(self._L.ss_replica_reset(self._h, rep))
@@ -261,7 +263,16 @@ class Engine:
         ids = np.ascontiguousarray(ids, np.int64)
         miss = ctypes.c_int64()
         self._ck(self._L.ss_halo_bind(self._h, rep, ptr(ids), ids.shape[0], ctypes.byref(miss)))
+        self._halo_n[rep] = ids.shape[0]
         return miss.value
+
+    def _halo_ticks(self, rep, ticks, buf, per_tick):
+        n = self._halo_n.get(rep)
+        if n is None:
+            raise ValueError(f"replica {rep} has no bound halo")
+        need = int(ticks) * per_tick(n)
+        if buf.size < need:
+            raise ValueError(f"{ticks} ticks of a {n}-record halo need {need} entries, got {buf.size}")
 
     def halo_unpack(self, rep, vals):
         vals = np.ascontiguousarray(vals, np.int8)
@@ -282,7 +293,28 @@ class Engine:
 
     def simulate_halo(self, rep, ticks, vals):
         vals = np.ascontiguousarray(vals, np.int8)
+        self._halo_ticks(rep, ticks, vals, lambda n: n)
         self._ck(self._L.ss_simulate_halo(self._h, rep, int(ticks), ptr(vals)))
+
+    # 2-bit halo: state codes packed four records a byte (pack_2bit below)
+    def halo_unpack_2bit(self, rep, packed, n):
+        packed = np.ascontiguousarray(packed, np.uint8)
+        if packed.size < (n + 3) // 4:
+            raise ValueError(f"{packed.size} bytes hold fewer than {n} records")
+        self._ck(self._L.ss_halo_unpack_2bit(self._h, rep, ptr(packed), int(n)))
+
+    def halo_pack_2bit(self, rep, slots):
+        slots = np.ascontiguousarray(slots, np.int32)
+        out = np.empty((slots.shape[0] + 3) // 4, np.uint8)
+        self._ck(self._L.ss_halo_pack_2bit(self._h, rep, ptr(slots), slots.shape[0], ptr(out)))
+        return out
+
+    def simulate_halo_2bit(self, rep, ticks, packed):
+        """simulate_halo with every tick's halo as 2-bit codes: packed[j] holds
+        tick j's records, (n_bound + 3) // 4 bytes (pack_2bit)."""
+        packed = np.ascontiguousarray(packed, np.uint8)
+        self._halo_ticks(rep, ticks, packed, lambda n: (n + 3) // 4)
+        self._ck(self._L.ss_simulate_halo_2bit(self._h, rep, int(ticks), ptr(packed)))
 
 
 @dataclass
@@ -377,3 +409,26 @@ class Replica:
 
     def remove(self):
         self.engine.remove_replica(self.rep)
+
+
+def pack_2bit(vals):
+    """State codes (< 4) to the 2-bit halo format along the last axis: record h
+    in bits 2 (h % 4) and 2 (h % 4) + 1 of byte h // 4."""
+    v = np.asarray(vals)
+    if v.size and (v.min() < 0 or v.max() > 3):
+        raise ValueError("2-bit halo codes must lie in [0, 4)")
+    n = v.shape[-1]
+    pad = (-n) % 4
+    u = v.astype(np.uint8)
+    if pad:
+        u = np.concatenate([u, np.zeros(v.shape[:-1] + (pad,), np.uint8)], axis=-1)
+    u = u.reshape(v.shape[:-1] + ((n + pad) // 4, 4))
+    return (u[..., 0] | (u[..., 1] << 2) | (u[..., 2] << 4) | (u[..., 3] << 6)).astype(np.uint8)
+
+
+def unpack_2bit(packed, n):
+    """Inverse of pack_2bit: the first n records as int8."""
+    p = np.asarray(packed, np.uint8)
+    u = np.stack([(p >> s) & 3 for s in (0, 2, 4, 6)], axis=-1).reshape(p.shape[:-1] + (4 * p.shape[-1],))
+    return u[..., :n].astype(np.int8)
+

@@ -248,3 +248,77 @@ def test_halo_from_page_locked_buffer(eng):
                         halo_vals=pw.halo_vals(0, pw.ticks))
     assert first_diff(out[1][0][:g.n], want[:g.n]) is None
     assert np.array_equal(out[1][1], hist)
+
+
+def test_halo_2bit_matches_bytes(eng):
+    """The 2-bit halo gives the states and history of the byte halo, from an
+    ordinary array (staged) and from page-locked memory, with a halo whose
+    record count is not a multiple of 4 and one ghost left unbound; and the
+    full binding matches the oracle."""
+    from sagesim_amd import pack_2bit
+    pw = workloads.sbm_partition(n_blocks=8, block_size=3000, deg_in=10, deg_out=2, n_parts=2, part=0,
+                                 ticks=24, seed=23)
+    g = pw.graph
+    for ids in (pw.halo_ids, pw.halo_ids[:-1] if pw.halo_ids.shape[0] % 4 != 1 else pw.halo_ids[:-2]):
+        n = ids.shape[0]
+        say("halo 2-bit", g.n, g.n_cols, "records", n)
+        vals = pw.halo_vals(0, pw.ticks)
+        if n < vals.shape[1]:
+            # records of the subset halo, in its order (vals columns follow pw.halo_ids)
+            vals = vals[:, :n]
+        out = []
+        for mode in ("bytes", "2bit", "2bit-pinned"):
+            rep = Replica(eng, g, pw.state, sir_model(pw.beta, pw.gamma), pw.seed, rng_key_id=pw.local_ids,
+                          history=pw.history)
+            eng.map_build(rep.rep, pw.ghost_ids, np.arange(g.n, g.n_cols, dtype=np.int32), 0.5)
+            assert eng.halo_bind(rep.rep, ids) == 0
+            for t in range(0, pw.ticks, 8):
+                v = vals[t:t + 8]
+                if mode == "bytes":
+                    eng.simulate_halo(rep.rep, 8, v)
+                elif mode == "2bit":
+                    eng.simulate_halo_2bit(rep.rep, 8, pack_2bit(v))
+                else:
+                    buf = eng.pinned_empty((8, (n + 3) // 4), np.uint8)
+                    buf[:] = pack_2bit(v)
+                    eng.simulate_halo_2bit(rep.rep, 8, buf)
+            out.append((rep.states(), rep.history()))
+            rep.remove()
+        for k in (1, 2):
+            assert first_diff(out[k][0], out[0][0]) is None
+            assert np.array_equal(out[k][1], out[0][1])
+        if n == pw.halo_ids.shape[0]:
+            hist = pw.history.copy()
+            want, _ = O.sir_run(O.sir_model(pw.beta, pw.gamma, pw.seed), g.row_ptr, g.col_idx, pw.local_ids,
+                                pw.state, pw.ticks, n_local=g.n, history=hist, ghost_ids=pw.ghost_ids,
+                                halo_ids=pw.halo_ids, halo_vals=pw.halo_vals(0, pw.ticks))
+            assert first_diff(out[1][0][:g.n], want[:g.n]) is None
+            assert np.array_equal(out[1][1], hist)
+
+
+def test_halo_2bit_pack_unpack_all_codes(eng):
+    """One 2-bit refresh of codes 0..3 (an SEIR replica) sets the ghosts as
+    the byte refresh does, and the 2-bit pack of slots equals the packed
+    byte pack."""
+    from sagesim_amd import pack_2bit
+    pw = workloads.sbm_partition(n_blocks=8, block_size=2000, deg_in=8, deg_out=2, n_parts=2, part=0,
+                                 ticks=4, seed=29)
+    g = pw.graph
+    n = pw.halo_ids.shape[0]
+    codes = np.random.default_rng(5).integers(0, 4, size=n).astype(np.int8)
+    got = []
+    for packed in (False, True):
+        rep = Replica(eng, g, pw.state, seir_model(0.05, 0.2, 0.1), pw.seed, rng_key_id=pw.local_ids)
+        eng.map_build(rep.rep, pw.ghost_ids, np.arange(g.n, g.n_cols, dtype=np.int32), 0.5)
+        assert eng.halo_bind(rep.rep, pw.halo_ids) == 0
+        if packed:
+            eng.halo_unpack_2bit(rep.rep, pack_2bit(codes), n)
+        else:
+            eng.halo_unpack(rep.rep, codes)
+        eng.simulate(2)
+        slots = np.random.default_rng(6).integers(0, g.n_cols, size=1001).astype(np.int32)
+        got.append((rep.states(), eng.halo_pack(rep.rep, slots), eng.halo_pack_2bit(rep.rep, slots)))
+        rep.remove()
+    assert first_diff(got[1][0], got[0][0]) is None
+    assert np.array_equal(got[1][2], pack_2bit(got[1][1]))
+    assert np.array_equal(got[0][2], got[1][2])

@@ -92,3 +92,22 @@ def test_footprint_is_positive_and_grows():
     b = L.ss_replica_footprint(2000, 0, 20_000, 0, 50)
     c = L.ss_replica_footprint(2000, 0, 20_000, 1, 50)
     assert 0 < a < b < c
+
+
+def test_2bit_halo_layout():
+    """Record h sits in bits 2 (h % 4) and 2 (h % 4) + 1 of byte h // 4; a
+    ragged tail is zero-padded; unpack inverts pack along the last axis."""
+    import numpy as np
+    from sagesim_amd import pack_2bit, unpack_2bit
+    # 1 | 2 << 2 | 3 << 4 | 0 << 6 = 57; the fifth record alone in byte 1
+    assert pack_2bit(np.array([1, 2, 3, 0, 1], np.int8)).tolist() == [57, 1]
+    assert pack_2bit(np.array([3, 3, 3, 3], np.int8)).tolist() == [255]
+    assert pack_2bit(np.zeros(0, np.int8)).shape == (0,)
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 3, 4, 5, 1023):
+        v = rng.integers(0, 4, size=(3, n)).astype(np.int8)
+        p = pack_2bit(v)
+        assert p.shape == (3, (n + 3) // 4) and p.dtype == np.uint8
+        assert np.array_equal(unpack_2bit(p, n), v)
+    with pytest.raises(ValueError):
+        pack_2bit(np.array([0, 4], np.int8))
