@@ -121,7 +121,8 @@ void free_replica(ss_replica &r) {
     dfree(r.history); dfree(r.init_history); dfree(r.inf_T);
     dfree(r.cnt[0]); dfree(r.cnt[1]); dfree(r.dec); dfree(r.acc);
     dfree(r.map_keys); dfree(r.map_vals); dfree(r.halo_slots);
-    dfree(r.halo_dev[0]); dfree(r.halo_dev[1]); dfree(r.halo_rec);
+    for (auto &d : r.halo_dev) dfree(d);
+    dfree(r.halo_rec);
     r = ss_replica();
 }
 
@@ -375,6 +376,15 @@ extern "C" {
 
 const char *ss_last_error(const ss_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
 
+static hipError_t create_halo_events(ss_ctx *ctx) {
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < SS_HALO_RING && e == hipSuccess; ++i) {
+        e = hipEventCreateWithFlags(&ctx->halo_ev[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->halo_used[i], hipEventDisableTiming);
+    }
+    return e;
+}
+
 int ss_create(int device, ss_ctx **out) {
     if (!out) SS_FAIL(nullptr, SS_ERR_INVALID, "out is NULL");
     *out = nullptr;
@@ -388,15 +398,12 @@ int ss_create(int device, ss_ctx **out) {
     if ((e = hipSetDevice(device)) != hipSuccess || (e = hipGetDeviceProperties(&prop, device)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&ctx->halo_ev[0], hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&ctx->halo_ev[1], hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&ctx->halo_used[0], hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&ctx->halo_used[1], hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreate(&ctx->t_start)) != hipSuccess || (e = hipEventCreate(&ctx->t_stop)) != hipSuccess ||
         (e = hipMalloc((void **)&ctx->d_work, SS_WORK_N * sizeof(unsigned long long))) != hipSuccess ||
         (e = hipMalloc((void **)&ctx->d_stats, 8 * sizeof(unsigned long long))) != hipSuccess ||
         (e = hipMemset(ctx->d_work, 0, SS_WORK_N * sizeof(unsigned long long))) != hipSuccess ||
-        (e = hipMalloc((void **)&ctx->d_counters, 64 * sizeof(int32_t))) != hipSuccess) {
+        (e = hipMalloc((void **)&ctx->d_counters, 64 * sizeof(int32_t))) != hipSuccess ||
+        (e = create_halo_events(ctx)) != hipSuccess) {
         delete ctx;
         SS_FAIL(nullptr, SS_ERR_HIP, "context setup: %s", hipGetErrorString(e));
     }
@@ -423,7 +430,7 @@ int ss_destroy(ss_ctx *ctx) {
     if (ctx->t_start) (void)hipEventDestroy(ctx->t_start);
     if (ctx->t_stop) (void)hipEventDestroy(ctx->t_stop);
     if (ctx->h_reps) (void)hipHostFree(ctx->h_reps);
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < SS_HALO_RING; ++i) {
         if (ctx->h_halo_pin[i]) (void)hipHostFree(ctx->h_halo_pin[i]);
         if (ctx->halo_ev[i]) (void)hipEventDestroy(ctx->halo_ev[i]);
         if (ctx->halo_used[i]) (void)hipEventDestroy(ctx->halo_used[i]);
@@ -896,13 +903,13 @@ int ss_halo_bind(ss_ctx *ctx, int32_t rep, const int64_t *ids, int64_t n, int64_
     ss_replica &r = ctx->reps[rep];
     if (!r.map_keys) SS_FAIL(ctx, SS_ERR_STATE, "map not built");
     SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    dfree(r.halo_slots); dfree(r.halo_dev[0]); dfree(r.halo_dev[1]); dfree(r.halo_rec);
-    r.halo_slots = nullptr; r.halo_dev[0] = r.halo_dev[1] = nullptr; r.halo_rec = nullptr;
+    dfree(r.halo_slots); dfree(r.halo_rec);
+    for (auto &d : r.halo_dev) { dfree(d); d = nullptr; }
+    r.halo_slots = nullptr; r.halo_rec = nullptr;
     r.n_halo = 0;
     if (n == 0) return SS_OK;
     SS_HIP(ctx, dalloc(ctx, &r, &r.halo_slots, (size_t)n));
-    SS_HIP(ctx, dalloc(ctx, &r, &r.halo_dev[0], (size_t)n));
-    SS_HIP(ctx, dalloc(ctx, &r, &r.halo_dev[1], (size_t)n));
+    for (auto &d : r.halo_dev) SS_HIP(ctx, dalloc(ctx, &r, &d, (size_t)n));
     int64_t miss = 0;
     int rc = lookup_dev(ctx, r, ids, r.halo_slots, n, &miss);
     if (rc) return rc;
@@ -1019,16 +1026,6 @@ static int simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *
     if (n == 0) return ss_simulate(ctx, ticks);
     int rc;
     if (ctx->tiles_dirty && (rc = build_tiles(ctx))) return rc;
-    if (ctx->h_halo_cap < n) {
-        SS_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));
-        for (int b = 0; b < 2; ++b) {
-            if (ctx->h_halo_pin[b]) (void)hipHostFree(ctx->h_halo_pin[b]);
-            ctx->h_halo_pin[b] = nullptr;
-        }
-        for (int b = 0; b < 2; ++b)
-            SS_HIP(ctx, hipHostMalloc((void **)&ctx->h_halo_pin[b], (size_t)n, hipHostMallocDefault));
-        ctx->h_halo_cap = n;
-    }
     if ((rc = push_descriptors(ctx, ticks))) return rc;
     // values in page-locked memory are copied from where they are
     bool pinned = false;
@@ -1038,19 +1035,30 @@ static int simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *
         pinned = e == hipSuccess && at.type == hipMemoryTypeHost;
         if (e != hipSuccess) (void)hipGetLastError();   // pageable memory: not an error here
     }
+    if (!pinned && ctx->h_halo_cap < n) {
+        SS_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));
+        for (auto &h : ctx->h_halo_pin) {
+            if (h) (void)hipHostFree(h);
+            h = nullptr;
+        }
+        ctx->h_halo_cap = 0;
+        for (auto &h : ctx->h_halo_pin) SS_HIP(ctx, hipHostMalloc((void **)&h, (size_t)n, hipHostMallocDefault));
+        ctx->h_halo_cap = n;
+    }
     const int64_t w0 = r.n_local / 64;
     for (int64_t k = 0; k < ticks; ++k) {
-        const int b = (int)(k & 1);
+        // ring slot b (device buffer, staging buffer, events) is free once the
+        // refresh that last read it has run: the host waits for that, so the
+        // copy stream never waits on the compute stream (a copy queued behind
+        // such a wait holds the calling thread until the wait resolves)
+        const int b = (int)(ctx->halo_pos++ % SS_HALO_RING);
+        SS_HIP(ctx, hipEventSynchronize(ctx->halo_used[b]));
         const int8_t *src = vals + (size_t)k * (size_t)n;
         if (!pinned) {
-            // the pinned slot is free once its previous copy finished
-            SS_HIP(ctx, hipEventSynchronize(ctx->halo_ev[b]));
             std::memcpy(ctx->h_halo_pin[b], src, (size_t)n);
             src = ctx->h_halo_pin[b];
         }
-        // device slot b is free once the refresh that last read it (two ticks
-        // back) has run; the copy then overlaps the previous tick's kernels
-        SS_HIP(ctx, hipStreamWaitEvent(ctx->copy_stream, ctx->halo_used[b], 0));
+        // the copy overlaps the kernels of the ticks before
         SS_HIP(ctx, hipMemcpyAsync(r.halo_dev[b], src, (size_t)n, hipMemcpyHostToDevice, ctx->copy_stream));
         SS_HIP(ctx, hipEventRecord(ctx->halo_ev[b], ctx->copy_stream));
         SS_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->halo_ev[b], 0));

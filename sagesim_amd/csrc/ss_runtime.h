@@ -9,6 +9,9 @@
 #include "ss_device.h"
 
 #define SS_BLOCK 256
+// device slots of a bound halo (and pinned staging buffers): the host runs at
+// most this many ticks ahead of the refreshes that read them
+#define SS_HALO_RING 4
 // work counters: S agents visited, draws, scans, CSR entries scanned,
 // counts pushed, replica-ticks taken in the push direction, agents of the
 // tiles that ran
@@ -95,7 +98,7 @@ struct ss_replica {
     int64_t n_halo = 0;
     int32_t *halo_rec = nullptr;                // record of slot halo_base + g, -1 when not bound
     int64_t halo_base = 0;                      // first slot refreshed (a multiple of 64)
-    int8_t *halo_dev[2] = {nullptr, nullptr};  // device staging of halo values
+    int8_t *halo_dev[SS_HALO_RING] = {};       // device staging of halo values, a ring by tick
     // bookkeeping
     int64_t tick = 0, tick_init = 0;
     int64_t tick_limit = INT64_MAX;            // replica finishes at this tick
@@ -137,10 +140,11 @@ struct ss_ctx {
     bool counting = false;
     uint32_t push_num = 4;                      // direction rule, 16 * ratio (0: pull only); ratio 0.25
     hipEvent_t t_start = nullptr, t_stop = nullptr;
-    int8_t *h_halo_pin[2] = {nullptr, nullptr}; // pinned staging for per-tick halo values
+    int8_t *h_halo_pin[SS_HALO_RING] = {};      // pinned staging for per-tick halo values
     int64_t h_halo_cap = 0;
-    hipEvent_t halo_ev[2] = {nullptr, nullptr};   // halo slot b copied to the device
-    hipEvent_t halo_used[2] = {nullptr, nullptr}; // the tick that read device halo slot b is done
+    hipEvent_t halo_ev[SS_HALO_RING] = {};      // ring slot b copied to the device
+    hipEvent_t halo_used[SS_HALO_RING] = {};    // the refresh that read ring slot b is done
+    int64_t halo_pos = 0;                       // ring position of the next halo tick
     std::string err;
     size_t bytes = 0;
 };
