@@ -88,23 +88,44 @@ static sg_graph *csr_from_pairs(int64_t n_rows, int64_t n_cols, const int32_t *p
     std::vector<int64_t> &rp = g->row_ptr;
     rp.assign(n_rows + 1, 0);
     int64_t *cnt = rp.data() + 1;
+    // one thread (a builder of a pool): plain increments; a locked add on a
+    // random cache line costs about as much as the miss itself
+    const bool serial = omp_get_max_threads() == 1;
+    if (serial) {
+        for (int64_t e = 0; e < ne; ++e) {
+            const int32_t u = pr[2 * e], v = pr[2 * e + 1];
+            if (u == v) continue;
+            if (u < n_rows) ++cnt[u];
+            if (!directed && v < n_rows) ++cnt[v];
+        }
+    } else {
 #pragma omp parallel for schedule(static)
-    for (int64_t e = 0; e < ne; ++e) {
-        const int32_t u = pr[2 * e], v = pr[2 * e + 1];
-        if (u == v) continue;
-        if (u < n_rows) __atomic_fetch_add(&cnt[u], 1, __ATOMIC_RELAXED);
-        if (!directed && v < n_rows) __atomic_fetch_add(&cnt[v], 1, __ATOMIC_RELAXED);
+        for (int64_t e = 0; e < ne; ++e) {
+            const int32_t u = pr[2 * e], v = pr[2 * e + 1];
+            if (u == v) continue;
+            if (u < n_rows) __atomic_fetch_add(&cnt[u], 1, __ATOMIC_RELAXED);
+            if (!directed && v < n_rows) __atomic_fetch_add(&cnt[v], 1, __ATOMIC_RELAXED);
+        }
     }
     for (int64_t i = 0; i < n_rows; ++i) rp[i + 1] += rp[i];
     std::vector<int64_t> cur(rp.begin(), rp.end() - 1);
     std::vector<int32_t> &col = g->col_idx;
     col.resize(rp[n_rows]);
+    if (serial) {
+        for (int64_t e = 0; e < ne; ++e) {
+            const int32_t u = pr[2 * e], v = pr[2 * e + 1];
+            if (u == v) continue;
+            if (u < n_rows) col[cur[u]++] = v;
+            if (!directed && v < n_rows) col[cur[v]++] = u;
+        }
+    } else {
 #pragma omp parallel for schedule(static)
-    for (int64_t e = 0; e < ne; ++e) {
-        const int32_t u = pr[2 * e], v = pr[2 * e + 1];
-        if (u == v) continue;
-        if (u < n_rows) col[__atomic_fetch_add(&cur[u], 1, __ATOMIC_RELAXED)] = v;
-        if (!directed && v < n_rows) col[__atomic_fetch_add(&cur[v], 1, __ATOMIC_RELAXED)] = u;
+        for (int64_t e = 0; e < ne; ++e) {
+            const int32_t u = pr[2 * e], v = pr[2 * e + 1];
+            if (u == v) continue;
+            if (u < n_rows) col[__atomic_fetch_add(&cur[u], 1, __ATOMIC_RELAXED)] = v;
+            if (!directed && v < n_rows) col[__atomic_fetch_add(&cur[v], 1, __ATOMIC_RELAXED)] = u;
+        }
     }
     // sort + unique every row (cur[i] := unique length), then compact left in place
 #pragma omp parallel for schedule(dynamic, 1024)
@@ -215,15 +236,34 @@ sg_graph *sg_ba(int64_t n, int32_t m, uint64_t seed) {
     int32_t tg[64];
     std::vector<int32_t> tgv;
     int32_t *t = m <= 64 ? tg : (tgv.resize(m), tgv.data());
+    // The draws come from a window of the stream's next outputs, taken in the
+    // stream's order (the graph is that of drawing one at a time).  Node
+    // v + D will take, barring redraws, the outputs D * m further on, and its
+    // list length is 2 m D more than now: its endpoints are prefetched D
+    // nodes ahead, which hides the cache misses of the random reads.
+    const int D = 4;
+    uint64_t qn = 1;
+    while (qn < (uint64_t)(D + 2) * (uint64_t)m + 1) qn <<= 1;
+    const uint64_t qm = qn - 1;
+    std::vector<uint64_t> q(qn);
+    uint64_t qh = 0, qt = 0;                   // outputs consumed / generated
+    auto pick = [](uint64_t x, uint64_t k) { return (uint64_t)(((unsigned __int128)x * (unsigned __int128)k) >> 64); };
     for (int64_t v = m + 1; v < n; ++v) {
+        while (qt < qh + (uint64_t)(D + 1) * (uint64_t)m) q[qt++ & qm] = r.next();
+        if (v + D < n) {
+            const uint64_t len_d = (uint64_t)len + 2ull * (uint64_t)m * D;
+            for (int32_t j = 0; j < m; ++j)
+                __builtin_prefetch(&rep[pick(q[(qh + (uint64_t)D * m + j) & qm], len_d)]);
+        }
         int cnt = 0;
         while (cnt < m) {
-            const int32_t c = rep[r.below((uint64_t)len)];
+            if (qh == qt) q[qt++ & qm] = r.next();
+            const int32_t c = rep[pick(q[qh++ & qm], (uint64_t)len)];
             bool dup = false;
-            for (int q = 0; q < cnt; ++q) dup |= t[q] == c;
+            for (int k = 0; k < cnt; ++k) dup |= t[k] == c;
             if (!dup) t[cnt++] = c;
         }
-        for (int q = 0; q < m; ++q) { rep[len++] = (int32_t)v; rep[len++] = t[q]; }
+        for (int k = 0; k < m; ++k) { rep[len++] = (int32_t)v; rep[len++] = t[k]; }
     }
     return csr_from_pairs(n, n, rep.data(), ne, false);
 }
