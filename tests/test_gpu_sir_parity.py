@@ -296,6 +296,44 @@ def test_halo_2bit_matches_bytes(eng):
             assert np.array_equal(out[1][1], hist)
 
 
+def test_halo_refresh_large_with_repeats(eng):
+    """A halo of 98k records in a random record order, then a binding with
+    three ghosts left out and five records repeated at the end with other
+    values (the later record of a slot wins): int8 and 2-bit halos against the
+    oracle's states and history, ghosts included."""
+    from sagesim_amd import pack_2bit
+    pw = workloads.sbm_partition(n_blocks=8, block_size=30000, deg_in=8, deg_out=3, n_parts=2, part=0,
+                                 ticks=12, seed=37)
+    g = pw.graph
+    base_vals = pw.halo_vals(0, pw.ticks)
+    cases = [(pw.halo_ids, base_vals)]
+    ids2 = np.concatenate([pw.halo_ids[:-3], pw.halo_ids[:5]])
+    vals2 = np.concatenate([base_vals[:, :-3], (base_vals[:, :5] + 1) % 3], axis=1).astype(np.int8)
+    cases.append((ids2, vals2))
+    for ids, vals in cases:
+        hist = pw.history.copy()
+        want, _ = O.sir_run(O.sir_model(pw.beta, pw.gamma, pw.seed), g.row_ptr, g.col_idx, pw.local_ids, pw.state,
+                            pw.ticks, n_local=g.n, history=hist, ghost_ids=pw.ghost_ids, halo_ids=ids, halo_vals=vals)
+        for packed in (False, True):
+            say("halo large", g.n, g.n_cols, "records", ids.shape[0], "packed", packed)
+            rep = Replica(eng, g, pw.state, sir_model(pw.beta, pw.gamma), pw.seed, rng_key_id=pw.local_ids,
+                          history=pw.history)
+            eng.map_build(rep.rep, pw.ghost_ids, np.arange(g.n, g.n_cols, dtype=np.int32), 0.5)
+            assert eng.halo_bind(rep.rep, ids) == 0
+            for t in range(0, pw.ticks, 4):
+                v = vals[t:t + 4]
+                if packed:
+                    eng.simulate_halo_2bit(rep.rep, 4, pack_2bit(v))
+                else:
+                    eng.simulate_halo(rep.rep, 4, v)
+            st, hi = rep.states(), rep.history()
+            rep.remove()
+            assert first_diff(st[:g.n], want[:g.n]) is None
+            assert np.array_equal(hi, hist)
+            # ghosts: the last tick's halo (later records win), the three unbound ones as in the oracle
+            assert first_diff(st[g.n:], want[g.n:]) is None
+
+
 def test_halo_2bit_pack_unpack_all_codes(eng):
     """One 2-bit refresh of codes 0..3 (an SEIR replica) sets the ghosts as
     the byte refresh does, and the 2-bit pack of slots equals the packed
