@@ -167,12 +167,13 @@ void drop_graphs(ss_ctx *ctx) {
 // Tiles of every live replica: one HUB tile per row of degree >= hub_deg at the
 // front, HEAVY tiles of SS_HEAVY_TILE agents up to the first 64-aligned slot
 // past the rows of degree >= heavy_deg, LIGHT tiles for the rest (light tiles
-// start 64-aligned: they write whole bitmap words, the hub and heavy tiles set
-// single bits).  A light tile holds 256 * apl agents; with 16 agents per lane
-// it holds 4 * 1024, fewer where its rows are longer than the replica's
-// average light row: a tile of rows of degree d > dbar holds dbar / d of that
-// (about as many CSR entries as an average tile), in steps of 256 agents, so
-// that the tiles at the long-row end cost about as much as the others.  Blocks stride over the list (see below).
+// start 64-aligned: no bitmap word is shared by two light tiles, the hub and
+// heavy tiles set single bits).  A light tile holds 256 * apl agents; with 16
+// agents per lane it holds 4 * 1024, fewer where its rows are longer than the
+// replica's average light row: a tile of rows of degree d > dbar holds dbar / d
+// of that (about as many CSR entries as an average tile), in steps of 256
+// agents down to 256 and of 64 below, so that the tiles at the long-row end
+// cost about as much as the others.  Blocks stride over the list (see below).
 int build_tiles(ss_ctx *ctx) {
     std::vector<ss_tile> hubs, heavy, light;
     int bpc = 0;
@@ -205,9 +206,11 @@ int build_tiles(ss_ctx *ctx) {
             const double d = (double)(E(g1) - E(b)) / (double)(g1 - b);
             int64_t a = cap;
             if (ctx->apl == 16 && dbar > 0.0 && d > dbar) a = (int64_t)((double)cap * dbar / d);
-            // whole groups of 4 x 64 agents: the 4 wavefronts of a wide tile get equal chunks
-            const int64_t step = ctx->apl == 16 ? 256 : 64;
-            a = std::max<int64_t>(step, std::min(a, cap) / step * step);
+            // whole groups of 4 x 64 agents: the 4 wavefronts of a wide tile get
+            // equal chunks of whole bitmap words; a wide tile of long rows may
+            // hold 64, 128 or 192 agents (chunks of 16, 32 or 48)
+            const int64_t step = ctx->apl == 16 && a >= 256 ? 256 : 64;
+            a = std::max<int64_t>(64, std::min(a, cap) / step * step);
             const int64_t e = std::min(b + a, nt);
             light.push_back({ri, SS_TILE_LIGHT, b, e, 0, 0});
             b = e;

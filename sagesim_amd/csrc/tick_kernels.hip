@@ -477,6 +477,13 @@ __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int
     }
 }
 
+// Bits of agents in the heavy region are set and cleared one agent at a time
+// (their words are shared by tiles that run concurrently).
+__device__ __forceinline__ void put_bits(SS_G uint64_t *nb, int64_t word, uint64_t set, uint64_t clr) {
+    if (set) atomicOr((unsigned long long *)&nb[word], (unsigned long long)set);
+    if (clr) atomicAnd((unsigned long long *)&nb[word], ~(unsigned long long)clr);
+}
+
 // Wide LIGHT tile: each wavefront owns a chunk of up to 1024 consecutive agents
 // and lane l the 16 agents [16 l, 16 l + 16) of it, so states, counts, degrees and
 // RNG keys arrive as 16-byte vector loads, one round trip for the chunk.  The
@@ -487,7 +494,7 @@ __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int
 // whole wavefront.  Results return through a byte per chunk slot.
 #define SS_WIDE_A 16
 #define SS_WIDE_CHUNK (64 * SS_WIDE_A)
-#define SS_WIDE_SHORT 128        // rows longer than this are scanned by the whole wavefront
+#define SS_WIDE_SHORT 512        // rows longer than this are scanned by the whole wavefront
 struct alignas(16) WideQueue {
     // draw entry: x = RNG key (pull) or infectious-neighbour count (push);
     //   y = slot | transition << 10 | (state & 3) << 11 | deg16 << 16 (deg16 only on pull)
@@ -532,8 +539,11 @@ __device__ __forceinline__ int64_t deg_of(const TickView &V, int64_t i, uint32_t
 __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int64_t begin, int64_t end, Work &w,
                                            Dir &dir, WideQueue &Q) {
     const int lane = opaque((int)threadIdx.x & 63), wv = threadIdx.x >> 6;
-    // the wavefront's chunk: a quarter of the tile rounded up to 64 agents (at most 1024)
-    const int64_t chunk = ((end - begin + 3) / 4 + 63) & ~(int64_t)63;
+    // the wavefront's chunk: a quarter of the tile rounded up to 64 agents (at
+    // most 1024), or to 16 in tiles of fewer than 256 agents (long rows), whose
+    // wavefronts then share bitmap words
+    const int64_t len4 = (end - begin + 3) / 4;
+    const int64_t chunk = end - begin < 256 ? (len4 + 15) & ~(int64_t)15 : (len4 + 63) & ~(int64_t)63;
     const int64_t c0 = begin + (int64_t)wv * chunk;
     if (c0 >= end) return;                                        // wave-uniform: no block barrier below
     if (end > c0 + chunk) end = c0 + chunk;
@@ -732,11 +742,18 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
 #pragma unroll 1
         for (int j = 0; j < nv; ++j) V.nxt[i0 + j] = (int8_t)((u4_word(nsv, j >> 2) >> (8 * (j & 3))) & 0xFFu);
     }
-    // infectious bits of the lane's agents; lanes 4g .. 4g+3 fill bitmap word g of the chunk
+    // infectious bits of the lane's agents; lanes 4g .. 4g+3 fill bitmap word g
+    // of a chunk of whole words, otherwise each lane sets and clears its own 16
+    // bits of a word it shares (chunk is wave-uniform)
     const uint32_t X = 0x01010101u * (uint32_t)(uint8_t)V.inf_code;
     const uint32_t ib = (eq_bytes4(nsv.x, X) | eq_bytes4(nsv.y, X) << 4 | eq_bytes4(nsv.z, X) << 8 |
                          eq_bytes4(nsv.w, X) << 12) & vmask;
-    {
+    if (chunk & 63) {
+        if (nv > 0) {
+            const int sh = (int)(i0 & 63);
+            put_bits(V.nb, i0 >> 6, (uint64_t)ib << sh, (uint64_t)(vmask & ~ib) << sh);
+        }
+    } else {
         uint32_t lo = (lane & 2) ? 0u : ib << (16 * (lane & 1));
         uint32_t hi = (lane & 2) ? ib << (16 * (lane & 1)) : 0u;
         lo |= (uint32_t)__shfl_xor((int32_t)lo, 1);
@@ -769,12 +786,6 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
     }
 }
 
-// Bits of agents in the heavy region are set and cleared one agent at a time
-// (their words are shared by tiles that run concurrently).
-__device__ __forceinline__ void put_bits(SS_G uint64_t *nb, int64_t word, uint64_t set, uint64_t clr) {
-    if (set) atomicOr((unsigned long long *)&nb[word], (unsigned long long)set);
-    if (clr) atomicAnd((unsigned long long *)&nb[word], ~(unsigned long long)clr);
-}
 
 // HEAVY tile: SS_HPW agents per wavefront.  Lanes 0..SS_HPW-1 load, draw and
 // decide for their agent in parallel; the rows that need a scan are then
