@@ -334,6 +334,30 @@ def test_halo_refresh_large_with_repeats(eng):
             assert first_diff(st[g.n:], want[g.n:]) is None
 
 
+def test_halo_buffers_too_short_are_refused(eng):
+    """simulate_halo and simulate_halo_2bit check the buffer against ticks x
+    the bound record count (int8 or packed) before the library reads it, and a
+    replica with no bound halo is refused."""
+    from sagesim_amd import pack_2bit
+    pw = workloads.sbm_partition(n_blocks=8, block_size=1000, deg_in=8, deg_out=2, n_parts=2, part=0,
+                                 ticks=4, seed=41)
+    g = pw.graph
+    rep = Replica(eng, g, pw.state, sir_model(pw.beta, pw.gamma), pw.seed, rng_key_id=pw.local_ids)
+    eng.map_build(rep.rep, pw.ghost_ids, np.arange(g.n, g.n_cols, dtype=np.int32), 0.5)
+    vals = pw.halo_vals(0, 4)
+    with pytest.raises(ValueError):
+        eng.simulate_halo(rep.rep, 2, vals[:2])        # no halo bound yet
+    assert eng.halo_bind(rep.rep, pw.halo_ids) == 0
+    with pytest.raises(ValueError):
+        eng.simulate_halo(rep.rep, 4, vals[:3])
+    with pytest.raises(ValueError):
+        eng.simulate_halo_2bit(rep.rep, 4, pack_2bit(vals[:3]))
+    with pytest.raises(ValueError):
+        eng.halo_unpack_2bit(rep.rep, pack_2bit(vals[0])[:-1], pw.halo_ids.shape[0])
+    eng.simulate_halo_2bit(rep.rep, 4, pack_2bit(vals))   # the full buffer runs
+    rep.remove()
+
+
 def test_halo_2bit_pack_unpack_all_codes(eng):
     """One 2-bit refresh of codes 0..3 (an SEIR replica) sets the ghosts as
     the byte refresh does, and the 2-bit pack of slots equals the packed
