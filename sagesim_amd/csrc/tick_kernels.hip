@@ -65,8 +65,6 @@ struct Cold {
     uint32_t T[SS_TLDS];         // inf_T[0 .. SS_TLDS)
     uint32_t tT[4];              // spontaneous-transition thresholds by state
     int32_t tN[4];               // ...and target states
-    unsigned long long *acc_next; // statistics of the next tick (null: replica not running)
-    const uint8_t *tcur;         // tile tags of this tick, null when every tile runs
     const uint32_t *inf_T;
     int32_t *history;
     uint16_t *pcnt;              // infectious-neighbour counts for the next tick
@@ -1055,6 +1053,10 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
     Dir dir;
     int cached = -1;
     bool live = false;
+    bool first = false;                              // the replica's first tile of this block
+    bool first_run = true;                           // ...and whether its tags let it run
+    const SS_G uint8_t *tcur = nullptr;              // tile tags of this tick, null when every tile runs
+    unsigned long long *acc_next = nullptr;          // statistics of the next tick
     TickView V;
     ss_tile nt{};
     if ((int)blockIdx.x < n_tiles) nt = tiles[blockIdx.x];
@@ -1063,17 +1065,35 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
         const int32_t rep = nt.rep, kind = nt.kind, tag_lo = nt.tag_lo, tag_hi = nt.tag_hi;
         const int64_t begin = nt.begin, end = nt.end;
         if (tix + (int)gridDim.x < n_tiles) nt = tiles[tix + gridDim.x];
+        first = false;
         if (rep != cached) {                         // block-uniform
-            if (cached >= 0) flush_dir(dir, C->acc_next, s_red);   // barriers inside: C is free after
+            if (cached >= 0) flush_dir(dir, acc_next, s_red);   // barriers inside: C is free after
             const ss_rep_dev *R = reps + rep;
             const int64_t t = R->tick0 + k;
             live = t < R->tick_end;
             if (live) {
                 const int32_t last = R->inf_T_last;
+                // what this tick needs from the replica beyond its descriptor,
+                // loaded before the barrier so that the loads overlap: the
+                // threshold table, the push decision of this tick (tags exist
+                // when the previous tick decided so), the rule for the next
+                // tick, and this tile's tags
+                const uint32_t tval =
+                    threadIdx.x < SS_TLDS ? R->inf_T[(int32_t)threadIdx.x < last ? (int32_t)threadIdx.x : last] : 0u;
+                const uint32_t dec = R->cnt[0] ? R->dec[t & 1] : 0u;
+                const bool produce = push_rule(R, t);
+                tcur = dec == 1u ? (const SS_G uint8_t *)R->tact[t & 1] : nullptr;
+                acc_next = R->acc ? acc_slot(R, t + 1) : nullptr;
+                V.tag = ss_front_tag(t);
+                first = true;
+                first_run = true;
+                if (tcur) {
+                    first_run = false;
+                    for (int32_t q = tag_lo; q <= tag_hi; ++q) first_run |= tcur[q] == V.tag;
+                }
                 __syncthreads();                     // the previous replica's fields are no longer read
                 const int64_t t1 = t + 1;
-                if (threadIdx.x < SS_TLDS)
-                    C->T[threadIdx.x] = R->inf_T[(int32_t)threadIdx.x < last ? (int32_t)threadIdx.x : last];
+                if (threadIdx.x < SS_TLDS) C->T[threadIdx.x] = tval;
                 if (threadIdx.x == 0) {
                     C->inf_T = R->inf_T;
                     C->history = R->history;
@@ -1101,14 +1121,9 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
                 V.cb = (const SS_G uint64_t *)(odd ? R->ibits[1] : R->ibits[0]);
                 V.nb = (SS_G uint64_t *)(odd ? R->ibits[0] : R->ibits[1]);
                 V.deg16 = (const SS_G uint16_t *)R->deg16;
-                // tags of this tick exist when the previous tick decided so;
                 // this tick writes the next tick's tags when the rule says so
-                const uint32_t dec = R->cnt[0] ? R->dec[t & 1] : 0u;
-                const bool produce = push_rule(R, t);
                 V.cnt = dec == 0u ? nullptr : (SS_G uint16_t *)R->cnt[t & 1];
                 V.push_next = produce ? 1 : 0;
-                V.tag = ss_front_tag(t);
-                const uint8_t *tcur = dec == 1u ? R->tact[t & 1] : nullptr;
                 if (tix == R->lead_tile && threadIdx.x == 0) {
                     // one block per replica: the decision for the next tick, the
                     // next tick's statistics started from this tick's totals
@@ -1132,20 +1147,17 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
                 V.t_lo = (uint32_t)t;
                 V.t_hi = (uint32_t)((uint64_t)t >> 32);
                 V.n_local = R->n_local;
-                if (threadIdx.x == 0) {
-                    C->acc_next = R->acc ? acc_slot(R, t + 1) : nullptr;
-                    C->tcur = tcur;
-                }
-                __syncthreads();
             }
             cached = rep;
         }
         if (!live) continue;                         // replica finished: block-uniform
         // push tick with tile tags: a tile without a tag has no candidate, no
         // agent that draws and the same states in both buffers (block-uniform)
-        if (C->tcur) {
+        if (first) {
+            if (!first_run) continue;
+        } else if (tcur) {
             bool run = false;
-            for (int32_t q = tag_lo; q <= tag_hi; ++q) run |= ((const SS_G uint8_t *)C->tcur)[q] == V.tag;
+            for (int32_t q = tag_lo; q <= tag_hi; ++q) run |= tcur[q] == V.tag;
             if (!run) continue;
         }
         if (threadIdx.x == 0) w.swept += (uint32_t)(end - begin);
@@ -1160,7 +1172,7 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
         else
             hub_tile(V, C, begin, tag_lo, w, dir, &s_cnt, &s_pos);
     }
-    if (cached >= 0) flush_dir(dir, C->acc_next, s_red);
+    if (cached >= 0) flush_dir(dir, acc_next, s_red);
     if constexpr (CNT) flush_work(counters, w);
 }
 
