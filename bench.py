@@ -414,7 +414,7 @@ def ensemble_leg(eng, args):
     eng.sync()
     t_up = time.perf_counter() - t0
     log(f"ensemble: {len(prepared)} replicas, {total} agents, built in {t_gen:.1f}s, uploaded in {t_up:.1f}s")
-    cands = [(h, hv, a) for h in (1024, 4096) for hv in (64, 128, 256) for a in (4, 8, 16)]
+    cands = [(h, hv, a) for h in (1024, 2048, 4096) for hv in (64, 128, 256) for a in (4, 8, 16)]
     best, res = autotune(eng, ens.reset, args.ensemble_ticks, candidates=cands)
     t_b = run_batched(eng, ens)
     batched = [ens.states(i) for i in range(len(prepared))]

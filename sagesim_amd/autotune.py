@@ -12,7 +12,7 @@ import itertools
 
 DEFAULT_CANDIDATES = [
     (hub, heavy, apl)
-    for hub, heavy, apl in itertools.product((512, 1024, 4096, 1 << 30), (32, 64, 128, 256, 512), (2, 4, 8, 16))
+    for hub, heavy, apl in itertools.product((512, 1024, 2048, 4096, 1 << 30), (32, 64, 128, 256, 512), (2, 4, 8, 16))
     if hub >= heavy
 ]
 
