@@ -107,6 +107,37 @@ def test_asymmetric_graph_stays_pull(engines):
     rep.remove()
 
 
+def test_seir_ba_bench_tiling_every_rule():
+    """SEIR on BA(150k, 4) with the tiling the 10M-agent bench picks (hub
+    2048, heavy 512, 16 agents per lane: hub, heavy and long-row light tiles
+    of 64-192 agents) under the three direction rules: the states after every
+    5th tick equal the oracle's trace."""
+    g = graphs.barabasi_albert(150_000, 4, 33)
+    st = workloads.seed_infected(g.n, 300, 34, code=2)
+    beta, sigma, gamma, seed = 0.06, 0.25, 0.12, 35
+    ticks = 90
+    _, trace = O.sir_run(O.seir_model(beta, sigma, gamma, seed), g.row_ptr, g.col_idx, np.arange(g.n), st, ticks,
+                         trace=True)
+    es = [Engine(0) for _ in RULES]
+    try:
+        reps = []
+        for e, r in zip(es, RULES):
+            e.set_push(r)
+            e.set_tiling(512, 16, 0, True, hub_degree=2048)
+            reps.append(Replica(e, g, st, seir_model(beta, sigma, gamma), seed))
+        say("seir ba bench tiling", g.n, g.nnz, [e.launch_shape() for e in es])
+        for t in range(ticks):
+            for e in es:
+                e.simulate(1)
+            if t % 5 == 4:
+                for r in reps:
+                    assert first_diff(r.states(), trace[t]) is None, (t, first_diff(r.states(), trace[t]))
+        assert (trace[-1] == 3).sum() > g.n // 20
+    finally:
+        for e in es:
+            e.close()
+
+
 def test_ensemble_mixed_directions(engines):
     """Replicas of different size, model, tick offset and symmetry in one
     launch, always-push rule: each equals its own oracle run."""
