@@ -15,4 +15,5 @@ from .space import NetworkSpace  # noqa: F401
 from .model import Model  # noqa: F401
 
 __version__ = "0.1.0"
-__all__ = ["Model", "Breed", "NetworkSpace", "StepFunction", "Engine", "Replica", "sir_model", "seir_model"]
+__all__ = ["Model", "Breed", "NetworkSpace", "StepFunction", "Engine", "Replica", "sir_model", "seir_model",
+           "pack_2bit", "unpack_2bit"]
