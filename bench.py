@@ -233,11 +233,29 @@ def main():
             log(f"cpu window ticks [{s0}, {s0 + win}): {dt:.2f}s, match {np.array_equal(got, want)}")
             if cpu_time > budget:
                 break
-        cpu = {"value": g.n * cpu_ticks / cpu_time, "unit": "agent-ticks/s", "cores": threads,
-               "kind": "port",
-               "sample": f"CPU oracle (C/OpenMP, {threads} threads, {cpu_model()}) on {cpu_ticks} ticks of the "
-                         f"same BA({g.n}, {args.m}) run: {win}-tick windows starting at ticks "
-                         f"{starts[:cpu_ticks // win]} from the GPU state; windows bit-exact vs GPU: {checked}"}
+        n_win = cpu_ticks // win
+        # then, budget permitting, the whole run from the initial state against the GPU's final states
+        full = None
+        if cpu_time < budget:
+            rep.reset()
+            st0 = rep.states()
+            eng.simulate(args.ticks)
+            want = rep.states()
+            c0 = time.perf_counter()
+            got, _ = O.sir_run(om, g.row_ptr, g.col_idx, ids, st0, args.ticks, tick0=0, nthreads=threads)
+            dt = time.perf_counter() - c0
+            cpu_time += dt
+            cpu_ticks += args.ticks
+            full = bool(np.array_equal(got, want))
+            log(f"cpu whole run, {args.ticks} ticks: {dt:.2f}s, final states match {full}")
+        sample = (f"CPU oracle (C/OpenMP, {threads} threads, {cpu_model()}) on {cpu_ticks} ticks of the "
+                  f"same BA({g.n}, {args.m}) run: {win}-tick windows starting at ticks {starts[:n_win]} from the "
+                  f"GPU state (bit-exact vs GPU: {checked})")
+        if full is not None:
+            sample += f", then the whole {args.ticks}-tick run from the initial state (final states bit-exact: {full})"
+        cpu = {"value": g.n * cpu_ticks / cpu_time, "unit": "agent-ticks/s", "cores": threads, "kind": "port",
+               "sample": sample, "seconds": round(cpu_time, 2), "windows_bitexact": checked,
+               "whole_run_bitexact": full}
 
     out = {
         "metric": "agent-ticks/s (plain SIR, BA 10M agents, 200 ticks)",
