@@ -35,6 +35,27 @@ def first_diff(a, b):
     return None if d.size == 0 else (int(d[0]), int(a[d[0]]), int(b[d[0]]), int(d.size))
 
 
+def test_ba_10m_whole_run_matches_oracle(eng):
+    """The headline workload at its full size: plain SIR on BA(10M, 5) for 200
+    ticks with the tiling the bench's autotuner picks (hub 2048, heavy 512, 16
+    agents per lane), final states equal to the oracle's whole run."""
+    import os
+    w = workloads.ba_sir(n=10_000_000)
+    say("ba 10M whole run", w.graph.n, w.graph.nnz)
+    eng.set_tiling(heavy_degree=512, agents_per_lane=16, hub_degree=2048)
+    try:
+        rep = Replica(eng, w.graph, w.state, w.model, w.seed)
+        eng.simulate(w.ticks)
+        got = rep.states()
+        rep.remove()
+    finally:
+        eng.set_tiling()
+    want, _ = O.sir_run(oracle_model(w), w.graph.row_ptr, w.graph.col_idx, np.arange(w.graph.n), w.state, w.ticks,
+                        nthreads=min(16, os.cpu_count() or 1))
+    assert first_diff(got, want) is None
+    assert np.bincount(want, minlength=3)[2] > w.graph.n // 2
+
+
 def test_er_sir_every_tick(eng):
     """10k-agent G(n, 0.001) SIR, 100 ticks: identical states after every tick."""
     w = workloads.er_sir()
