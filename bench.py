@@ -305,8 +305,8 @@ def partition_leg(eng, args):
     host link as 2-bit state codes (ss_simulate_halo_2bit, packed on the host
     before the timed region).  One untimed 10-tick call warms the launch
     path, then the replica is reset; the run is timed in 10-tick calls, halo
-    upload included, and its first 10 ticks are checked against the oracle
-    (states and history)."""
+    upload included; its first 10 ticks, and then the whole run, are checked
+    against the oracle (states and history)."""
     from sagesim_amd import Replica, pack_2bit, sir_model, workloads
     t0 = time.perf_counter()
     pw = workloads.sbm_partition(ticks=args.partition_ticks)
@@ -349,6 +349,17 @@ def partition_leg(eng, args):
                                 halo_ids=pw.halo_ids, halo_vals=vals, nthreads=host_threads())
             exact = bool(np.array_equal(rep.states()[:g.n], want[:g.n]) and np.array_equal(rep.history(), hist))
     final = np.bincount(rep.states()[:g.n], minlength=3).tolist()
+    whole = None
+    if not args.no_cpu:
+        # the whole run on the oracle, fed the same halo (as int8 codes): final states and history
+        from oracle import oracle as O
+        c0 = time.perf_counter()
+        hist = pw.history.copy()
+        want, _ = O.sir_run(O.sir_model(pw.beta, pw.gamma, pw.seed), g.row_ptr, g.col_idx, pw.local_ids, pw.state,
+                            pw.ticks, n_local=g.n, history=hist, ghost_ids=pw.ghost_ids, halo_ids=pw.halo_ids,
+                            halo_vals=pw.halo_vals(0, pw.ticks), nthreads=host_threads())
+        whole = bool(np.array_equal(rep.states()[:g.n], want[:g.n]) and np.array_equal(rep.history(), hist))
+        log(f"partition: whole {pw.ticks}-tick run on the oracle in {time.perf_counter() - c0:.1f}s, exact {whole}")
     rep.remove()
     rate = g.n * pw.ticks / t_run
     log(f"partition: {pw.ticks} ticks in {t_run * 1e3:.1f} ms ({rate:.3g} local agent-ticks/s), "
@@ -359,7 +370,8 @@ def partition_leg(eng, args):
             "halo_ids_missing": int(miss), "seconds": round(t_run, 4), "ms_per_tick": round(t_run * 1e3 / pw.ticks, 4),
             "local_agent_ticks_per_s": rate, "halo_records": int(n_halo),
             "halo_bytes_per_tick": int((n_halo + 3) // 4), "halo_format": "2-bit state codes, four records a byte",
-            "oracle_window_ticks": chunk, "oracle_window_exact": exact, "final_SIR": final,
+            "oracle_window_ticks": chunk, "oracle_window_exact": exact, "oracle_whole_run_exact": whole,
+            "final_SIR": final,
             "graph_build_s": round(t_gen, 2), "upload_s": round(t_up, 2)}
 
 
