@@ -421,11 +421,19 @@ def stepfunc_leg(args):
     m.simulate(ticks)
     dt = time.perf_counter() - c0
     rate = n * ticks / dt
+    whole = None
+    if not args.no_cpu:
+        # all 10 + ticks ticks on the oracle from the initial states
+        want = O.ref_sir_run(g.row_ptr, g.col_idx, np.arange(n, dtype=np.float32), init.astype(np.float32), seed,
+                             10 + ticks, 0.03, 0.1)
+        got = np.asarray(m._device_property("state"), np.float32).reshape(-1)[m._inv]
+        whole = bool(np.array_equal(got, want))
     log(f"stepfunc: BA({n}, 5), {ticks} ticks of the hipRTC-compiled reference step in {dt * 1e3:.1f} ms "
-        f"({rate:.3g} agent-ticks/s), mode {m.mode}, oracle window exact {exact}")
+        f"({rate:.3g} agent-ticks/s), mode {m.mode}, oracle window exact {exact}, whole run exact {whole}")
     return {"workload": f"reference SIR step function (hipRTC), BA({n}, 5), float32 states, Philox-2x32",
             "agents": n, "nnz": int(g.nnz), "ticks": ticks, "seconds": round(dt, 4), "agent_ticks_per_s": rate,
             "mode": m.mode, "oracle_window_ticks": 10, "oracle_window_exact": exact,
+            "oracle_whole_run_ticks": 10 + ticks, "oracle_whole_run_exact": whole,
             "build_s": round(t_build, 2), "setup_s": round(t_setup, 2)}
 
 
