@@ -131,11 +131,35 @@ int ss_launch_shape(ss_ctx *ctx, int32_t *n_tiles, int32_t *grid);
 /* ---- ghosts of one partition: replace GPUHashMap (gpu_kernels.py:91-260,
  *      which the reference builds but never queries) and the pack/unpack of
  *      CommunicationManager (gpu_kernels.py:852-909, :984-1066) ---- */
-/* (Re)build the id -> slot map; capacity = max(64, ceil(n / load_factor)). */
+/* The map's arrays are laid out as the reference's GPUHashMap leaves its CPU
+ * mirror after the same calls: keys int64 (-1 empty, -2 deleted), values
+ * int32 (-1 in empty and deleted slots), home slot floor-mod(id * 2654435761
+ * wrapped to int64, capacity), linear probing.
+ * ss_map_build: GPUHashMap(capacity).build_from_arrays(ids, slots)
+ *   (gpu_kernels.py:122-169, capacity at model.py:1113); capacity =
+ *   max(64, ceil(n / load_factor)); the reference's placement rounds, so a
+ *   repeated id is stored once per occurrence and lookups find the first
+ *   occurrence in input order; the size counts every entry.
+ * ss_map_insert: GPUHashMap.insert (gpu_kernels.py:171-191) for every entry
+ *   in input order: the probe stops at an empty or deleted slot or at the
+ *   id's own key; the value of an existing key is overwritten.  An insert
+ *   that finds no slot in a whole sweep of a full table returns SS_ERR_FULL
+ *   (the reference would probe forever); grow the map with ss_map_resize, as
+ *   the reference's callers are to do when needs_resize() (size > 0.7 *
+ *   capacity, gpu_kernels.py:217-219).
+ * ss_map_remove: GPUHashMap.remove (gpu_kernels.py:193-206) in input order;
+ *   *n_removed counts the entries marked deleted. */
 int ss_map_build(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int32_t *slots, int64_t n,
                  double load_factor);
 int ss_map_insert(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int32_t *slots, int64_t n);
 int ss_map_remove(ss_ctx *ctx, int32_t rep, const int64_t *ids, int64_t n, int64_t *n_removed);
+/* GPUHashMap.resize (gpu_kernels.py:221-240): a table of new_capacity slots
+ * with every live entry re-inserted in the old table's slot order. */
+int ss_map_resize(ss_ctx *ctx, int32_t rep, int64_t new_capacity);
+/* GPUHashMap.capacity and .size (gpu_kernels.py:103-106) */
+int ss_map_info(ss_ctx *ctx, int32_t rep, int64_t *capacity, int64_t *size);
+/* copies of GPUHashMap.keys / .values (capacity entries each) */
+int ss_map_download(ss_ctx *ctx, int32_t rep, int64_t *keys, int32_t *vals, int64_t capacity);
 /* out[i] = slot of ids[i] or -1 (GPUHashMap.lookup_cpu, gpu_kernels.py:208-215) */
 int ss_map_lookup(ss_ctx *ctx, int32_t rep, const int64_t *ids, int32_t *out, int64_t n, int64_t *n_missing);
 /* Fix the record order of the halo buffer: record h refreshes the ghost with

@@ -89,6 +89,9 @@ def step_lib():
     _proto(L, "ss_map_insert", I, _P, _c_i32, _P, _P, _c_i64)
     _proto(L, "ss_map_remove", I, _P, _c_i32, _P, _c_i64, _P)
     _proto(L, "ss_map_lookup", I, _P, _c_i32, _P, _P, _c_i64, _P)
+    _proto(L, "ss_map_resize", I, _P, _c_i32, _c_i64)
+    _proto(L, "ss_map_info", I, _P, _c_i32, _P, _P)
+    _proto(L, "ss_map_download", I, _P, _c_i32, _P, _P, _c_i64)
     _proto(L, "ss_halo_bind", I, _P, _c_i32, _P, _c_i64, _P)
     _proto(L, "ss_halo_unpack", I, _P, _c_i32, _P, _c_i64)
     _proto(L, "ss_halo_unpack_ids", I, _P, _c_i32, _P, _P, _c_i64, _P)

@@ -795,78 +795,180 @@ int ss_simulate(ss_ctx *ctx, int64_t ticks) {
 
 // ---------------- ghosts ----------------
 
-int ss_map_build(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int32_t *slots, int64_t n, double lf) {
-    if (!valid_rep(ctx, rep) || (n && (!ids || !slots))) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
-    if (!(lf > 0.0 && lf <= 0.9)) SS_FAIL(ctx, SS_ERR_INVALID, "load_factor must be in (0, 0.9]");
-    ss_replica &r = ctx->reps[rep];
+static int check_map_entries(ss_ctx *ctx, const ss_replica &r, const int64_t *ids, const int32_t *slots,
+                             int64_t n) {
     for (int64_t i = 0; i < n; ++i) {
         if (ids[i] == -1 || ids[i] == -2) SS_FAIL(ctx, SS_ERR_RANGE, "ids -1 and -2 are reserved");
-        if (slots[i] < 0 || slots[i] >= r.n_total) SS_FAIL(ctx, SS_ERR_RANGE, "slot %d outside the replica", slots[i]);
+        if (slots && (slots[i] < 0 || slots[i] >= r.n_total))
+            SS_FAIL(ctx, SS_ERR_RANGE, "slot %d outside the replica", slots[i]);
     }
-    int64_t cap = std::max<int64_t>(64, (int64_t)std::ceil((double)n / lf));
-    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (cap != r.map_cap) {
-        if (r.map_keys) { r.bytes -= sizeof(unsigned long long) * r.map_cap + sizeof(int32_t) * r.map_cap; }
+    return SS_OK;
+}
+
+// device copies of a batch of map entries (freed by the destructor)
+struct MapBatch {
+    int64_t *ids = nullptr;
+    int32_t *vals = nullptr;
+    ~MapBatch() { dfree(ids); dfree(vals); }
+    hipError_t stage(const int64_t *h_ids, const int32_t *h_vals, int64_t n) {
+        hipError_t e = hipMalloc((void **)&ids, sizeof(int64_t) * (size_t)n);
+        if (e == hipSuccess && h_vals) e = hipMalloc((void **)&vals, sizeof(int32_t) * (size_t)n);
+        if (e == hipSuccess) e = hipMemcpy(ids, h_ids, sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice);
+        if (e == hipSuccess && h_vals) e = hipMemcpy(vals, h_vals, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice);
+        return e;
+    }
+};
+
+static int alloc_map(ss_ctx *ctx, ss_replica &r, int64_t cap) {
+    if (cap != r.map_cap || !r.map_keys) {
+        if (r.map_keys) {
+            const size_t b = (sizeof(unsigned long long) + sizeof(int32_t)) * (size_t)r.map_cap;
+            r.bytes -= b;
+            ctx->bytes -= b;
+        }
         dfree(r.map_keys); dfree(r.map_vals);
-        r.map_keys = nullptr; r.map_vals = nullptr;
+        r.map_keys = nullptr; r.map_vals = nullptr; r.map_cap = 0;
         SS_HIP(ctx, dalloc(ctx, &r, &r.map_keys, (size_t)cap));
         SS_HIP(ctx, dalloc(ctx, &r, &r.map_vals, (size_t)cap));
         r.map_cap = cap;
     }
     SS_HIP(ctx, ss_launch_map_clear(r.map_keys, r.map_vals, cap, ctx->stream));
     r.map_size = 0;
-    return ss_map_insert(ctx, rep, ids, slots, n);
+    return SS_OK;
+}
+
+// entries of a batch in input order on one wavefront; out[0] = entries
+// added or removed, out[1] = entries that found no slot
+static int map_seq(ss_ctx *ctx, ss_replica &r, int op, const int64_t *ids, const int32_t *vals, int64_t n,
+                   int32_t out[2]) {
+    out[0] = out[1] = 0;
+    if (n == 0) return SS_OK;
+    MapBatch b;
+    hipError_t e = b.stage(ids, vals, n);
+    if (e == hipSuccess) e = hipMemsetAsync(ctx->d_counters, 0, 2 * sizeof(int32_t), ctx->stream);
+    if (e == hipSuccess)
+        e = ss_launch_map_seq(r.map_keys, r.map_vals, r.map_cap, op, b.ids, b.vals, n, ctx->d_counters, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, ctx->d_counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost);
+    SS_HIP(ctx, e);
+    return SS_OK;
+}
+
+int ss_map_build(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int32_t *slots, int64_t n, double lf) {
+    if (!valid_rep(ctx, rep) || n < 0 || (n && (!ids || !slots))) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    if (!(lf > 0.0 && lf <= 0.9)) SS_FAIL(ctx, SS_ERR_INVALID, "load_factor must be in (0, 0.9]");
+    if (n >= INT32_MAX) SS_FAIL(ctx, SS_ERR_INVALID, "at most 2^31 - 2 entries");
+    ss_replica &r = ctx->reps[rep];
+    int rc = check_map_entries(ctx, r, ids, slots, n);
+    if (rc) return rc;
+    const int64_t cap = std::max<int64_t>(64, (int64_t)std::ceil((double)n / lf));
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if ((rc = alloc_map(ctx, r, cap))) return rc;
+    if (n > 0) {
+        MapBatch b;
+        int64_t *probe = nullptr;
+        int32_t *claim = nullptr;
+        hipError_t e = b.stage(ids, slots, n);
+        if (e == hipSuccess) e = hipMalloc((void **)&probe, sizeof(int64_t) * (size_t)n);
+        if (e == hipSuccess) e = hipMalloc((void **)&claim, sizeof(int32_t) * (size_t)cap);
+        if (e == hipSuccess) e = ss_launch_map_build_init(b.ids, n, cap, probe, claim, ctx->stream);
+        // rounds in batches of four between reads of the count left unplaced;
+        // every round places at least one entry, so n + cap rounds bound it
+        const int R = 4;
+        int64_t rounds = 0;
+        int32_t left = 1;
+        while (e == hipSuccess && left > 0 && rounds <= n + cap) {
+            e = hipMemsetAsync(ctx->d_counters, 0, R * sizeof(int32_t), ctx->stream);
+            for (int j = 0; j < R && e == hipSuccess; ++j)
+                e = ss_launch_map_build_round(r.map_keys, r.map_vals, cap, b.ids, b.vals, n, probe, claim,
+                                              ctx->d_counters + j, ctx->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(&left, ctx->d_counters + R - 1, sizeof(int32_t),
+                                                    hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            rounds += R;
+        }
+        dfree(probe);
+        dfree(claim);
+        SS_HIP(ctx, e);
+        if (left > 0) SS_FAIL(ctx, SS_ERR_FULL, "map build left %d entries unplaced", left);
+    }
+    r.map_size = n;    // gpu_kernels.py:137 counts every entry, repeated ids included
+    return SS_OK;
 }
 
 int ss_map_insert(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int32_t *slots, int64_t n) {
-    if (!valid_rep(ctx, rep) || (n && (!ids || !slots))) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    if (!valid_rep(ctx, rep) || n < 0 || (n && (!ids || !slots))) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
     ss_replica &r = ctx->reps[rep];
     if (!r.map_keys) SS_FAIL(ctx, SS_ERR_STATE, "map not built");
-    if (n == 0) return SS_OK;
-    for (int64_t i = 0; i < n; ++i) {
-        if (ids[i] == -1 || ids[i] == -2) SS_FAIL(ctx, SS_ERR_RANGE, "ids -1 and -2 are reserved");
-        if (slots[i] < 0 || slots[i] >= r.n_total) SS_FAIL(ctx, SS_ERR_RANGE, "slot %d outside the replica", slots[i]);
-    }
-    if (r.map_size + n > r.map_cap) SS_FAIL(ctx, SS_ERR_FULL, "map capacity %lld exceeded", (long long)r.map_cap);
-    int64_t *d_ids = nullptr;
-    int32_t *d_slots = nullptr;
-    SS_HIP(ctx, hipMalloc((void **)&d_ids, sizeof(int64_t) * n));
-    hipError_t e = hipMalloc((void **)&d_slots, sizeof(int32_t) * n);
-    if (e != hipSuccess) { dfree(d_ids); SS_HIP(ctx, e); }
-    int32_t zero = 0, overflow = 0;
-    e = hipMemcpy(d_ids, ids, sizeof(int64_t) * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_slots, slots, sizeof(int32_t) * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(ctx->d_counters, &zero, sizeof(int32_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = ss_launch_map_insert(r.map_keys, r.map_vals, r.map_cap, d_ids, d_slots, n, ctx->d_counters, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e == hipSuccess) e = hipMemcpy(&overflow, ctx->d_counters, sizeof(int32_t), hipMemcpyDeviceToHost);
-    dfree(d_ids);
-    dfree(d_slots);
-    SS_HIP(ctx, e);
-    if (overflow) SS_FAIL(ctx, SS_ERR_FULL, "%d inserts found no free slot", overflow);
-    r.map_size += n;
+    int rc = check_map_entries(ctx, r, ids, slots, n);
+    if (rc) return rc;
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int32_t out[2];
+    if ((rc = map_seq(ctx, r, SS_MAP_INSERT, ids, slots, n, out))) return rc;
+    r.map_size += out[0];
+    if (out[1]) SS_FAIL(ctx, SS_ERR_FULL, "%d inserts found no free slot (capacity %lld; see ss_map_resize)",
+                        out[1], (long long)r.map_cap);
     return SS_OK;
 }
 
 int ss_map_remove(ss_ctx *ctx, int32_t rep, const int64_t *ids, int64_t n, int64_t *n_removed) {
-    if (!valid_rep(ctx, rep) || (n && !ids)) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    if (!valid_rep(ctx, rep) || n < 0 || (n && !ids)) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
     ss_replica &r = ctx->reps[rep];
     if (!r.map_keys) SS_FAIL(ctx, SS_ERR_STATE, "map not built");
-    int32_t removed = 0;
-    if (n > 0) {
-        int64_t *d_ids = nullptr;
-        SS_HIP(ctx, hipMalloc((void **)&d_ids, sizeof(int64_t) * n));
-        int32_t zero = 0;
-        hipError_t e = hipMemcpy(d_ids, ids, sizeof(int64_t) * n, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(ctx->d_counters, &zero, sizeof(int32_t), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = ss_launch_map_remove(r.map_keys, r.map_vals, r.map_cap, d_ids, n, ctx->d_counters, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e == hipSuccess) e = hipMemcpy(&removed, ctx->d_counters, sizeof(int32_t), hipMemcpyDeviceToHost);
-        dfree(d_ids);
-        SS_HIP(ctx, e);
-    }
-    r.map_size -= removed;
-    if (n_removed) *n_removed = removed;
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int32_t out[2];
+    int rc = map_seq(ctx, r, SS_MAP_REMOVE, ids, nullptr, n, out);
+    if (rc) return rc;
+    r.map_size -= out[0];
+    if (n_removed) *n_removed = out[0];
+    return SS_OK;
+}
+
+int ss_map_resize(ss_ctx *ctx, int32_t rep, int64_t new_cap) {
+    if (!valid_rep(ctx, rep)) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica");
+    ss_replica &r = ctx->reps[rep];
+    if (!r.map_keys) SS_FAIL(ctx, SS_ERR_STATE, "map not built");
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // live entries in slot order (gpu_kernels.py:232-234)
+    std::vector<int64_t> keys((size_t)r.map_cap);
+    std::vector<int32_t> vals((size_t)r.map_cap);
+    SS_HIP(ctx, hipMemcpy(keys.data(), r.map_keys, sizeof(int64_t) * keys.size(), hipMemcpyDeviceToHost));
+    SS_HIP(ctx, hipMemcpy(vals.data(), r.map_vals, sizeof(int32_t) * vals.size(), hipMemcpyDeviceToHost));
+    size_t live = 0;
+    for (size_t s = 0; s < keys.size(); ++s)
+        if (keys[s] != -1 && keys[s] != -2) {
+            keys[live] = keys[s];
+            vals[live] = vals[s];
+            ++live;
+        }
+    if (new_cap < 1 || new_cap < (int64_t)live || new_cap >= INT32_MAX)
+        SS_FAIL(ctx, SS_ERR_INVALID, "capacity %lld cannot hold %zu entries", (long long)new_cap, live);
+    int rc = alloc_map(ctx, r, new_cap);
+    if (rc) return rc;
+    int32_t out[2];
+    if ((rc = map_seq(ctx, r, SS_MAP_RAW, keys.data(), vals.data(), (int64_t)live, out))) return rc;
+    r.map_size = out[0];
+    if (out[1]) SS_FAIL(ctx, SS_ERR_FULL, "%d entries found no slot", out[1]);
+    return SS_OK;
+}
+
+int ss_map_info(ss_ctx *ctx, int32_t rep, int64_t *capacity, int64_t *size) {
+    if (!valid_rep(ctx, rep)) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica");
+    const ss_replica &r = ctx->reps[rep];
+    if (capacity) *capacity = r.map_keys ? r.map_cap : 0;
+    if (size) *size = r.map_keys ? r.map_size : 0;
+    return SS_OK;
+}
+
+int ss_map_download(ss_ctx *ctx, int32_t rep, int64_t *keys, int32_t *vals, int64_t capacity) {
+    if (!valid_rep(ctx, rep) || !keys || !vals) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
+    const ss_replica &r = ctx->reps[rep];
+    if (!r.map_keys) SS_FAIL(ctx, SS_ERR_STATE, "map not built");
+    if (capacity != r.map_cap) SS_FAIL(ctx, SS_ERR_INVALID, "capacity %lld, map has %lld", (long long)capacity,
+                                       (long long)r.map_cap);
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SS_HIP(ctx, hipMemcpy(keys, r.map_keys, sizeof(int64_t) * (size_t)capacity, hipMemcpyDeviceToHost));
+    SS_HIP(ctx, hipMemcpy(vals, r.map_vals, sizeof(int32_t) * (size_t)capacity, hipMemcpyDeviceToHost));
     return SS_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "ss_device.h"
+#include "ss_map.h"
 
 #define SS_BLOCK 256
 // device slots of a bound halo (and pinned staging buffers): the host runs at
@@ -30,13 +31,8 @@ static const int32_t SS_THRESHOLDS[SS_N_THRESHOLDS] = {8, 16, 24, 32, 48, 64, 96
 hipError_t ss_occupancy_tick(int apl, int *blocks_per_cu);
 hipError_t ss_launch_bits(const int8_t *st, uint64_t *bits, int64_t w0, int64_t w1, int64_t n_total, int8_t inf,
                           hipStream_t s);
-hipError_t ss_launch_map_clear(unsigned long long *keys, int32_t *vals, int64_t cap, hipStream_t s);
-hipError_t ss_launch_map_insert(unsigned long long *keys, int32_t *vals, int64_t cap, const int64_t *ids,
-                                const int32_t *slots, int64_t n, int32_t *overflow, hipStream_t s);
 hipError_t ss_launch_map_lookup(const unsigned long long *keys, const int32_t *vals, int64_t cap, const int64_t *ids,
                                 int32_t *out, int64_t n, int32_t *misses, hipStream_t s);
-hipError_t ss_launch_map_remove(unsigned long long *keys, int32_t *vals, int64_t cap, const int64_t *ids, int64_t n,
-                                int32_t *removed, hipStream_t s);
 // ghost refresh from a bound halo: slot base + g <- vals[rec[g]] (rec -1: kept),
 // both buffers, and the bitmap words of [base, n_total) of the buffer `cur`;
 // packed: vals holds 2-bit codes, four records a byte

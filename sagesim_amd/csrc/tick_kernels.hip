@@ -1296,45 +1296,11 @@ __global__ void bits_kernel(const int8_t *__restrict__ st, uint64_t *__restrict_
     if ((threadIdx.x & 63) == 0 && (i >> 6) < w1) bits[i >> 6] = word;
 }
 
-// ---- open-addressing id -> slot map (gpu_kernels.py:91-260 layout) ----
-// slot of id: floor-mod of the wrapped int64 product id * 2654435761 by the
-// capacity, as the reference's numpy expression computes it
-// (gpu_kernels.py:112-114, :146); linear probing; key -1 empty, -2 deleted.
-__device__ __forceinline__ int64_t map_home(int64_t id, int64_t cap) {
-    const int64_t h = (int64_t)((uint64_t)id * 2654435761ull);
-    int64_t s = h % cap;
-    return s < 0 ? s + cap : s;
-}
-
-__global__ void map_insert_kernel(unsigned long long *keys, int32_t *vals, int64_t cap,
-                                  const int64_t *__restrict__ ids, const int32_t *__restrict__ slots,
-                                  int64_t n, int32_t *overflow) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    const int64_t id = ids[g];
-    int64_t s = map_home(id, cap);
-    for (int64_t probe = 0; probe < cap; ++probe) {
-        unsigned long long prev = atomicCAS(&keys[s], (unsigned long long)(-1ll), (unsigned long long)id);
-        if (prev == (unsigned long long)(-1ll) || prev == (unsigned long long)id) {
-            // equal keys: the lowest slot value wins, independent of thread order
-            atomicMin(&vals[s], slots[g]);
-            return;
-        }
-        if (prev == (unsigned long long)(-2ll)) {
-            prev = atomicCAS(&keys[s], (unsigned long long)(-2ll), (unsigned long long)id);
-            if (prev == (unsigned long long)(-2ll) || prev == (unsigned long long)id) {
-                atomicMin(&vals[s], slots[g]);
-                return;
-            }
-        }
-        s = s + 1 == cap ? 0 : s + 1;
-    }
-    atomicAdd(overflow, 1);
-}
-
+// ---- id -> slot lookups (gpu_kernels.py:208-215): probe from the home slot
+// until the id or an empty slot ----
 __device__ __forceinline__ int32_t map_find(const unsigned long long *__restrict__ keys,
                                             const int32_t *__restrict__ vals, int64_t cap, int64_t id) {
-    int64_t s = map_home(id, cap);
+    int64_t s = ss_map_home(id, cap);
     for (int64_t probe = 0; probe < cap; ++probe) {
         const unsigned long long k = keys[s];
         if (k == (unsigned long long)id) return vals[s];
@@ -1354,30 +1320,6 @@ __global__ void map_lookup_kernel(const unsigned long long *__restrict__ keys, c
     if (v < 0 && misses) atomicAdd(misses, 1);
 }
 
-__global__ void map_remove_kernel(unsigned long long *keys, int32_t *vals, int64_t cap,
-                                  const int64_t *__restrict__ ids, int64_t n, int32_t *removed) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    const int64_t id = ids[g];
-    int64_t s = map_home(id, cap);
-    for (int64_t probe = 0; probe < cap; ++probe) {
-        const unsigned long long k = keys[s];
-        if (k == (unsigned long long)(-1ll)) return;
-        if (k == (unsigned long long)id) {
-            if (atomicCAS(&keys[s], (unsigned long long)id, (unsigned long long)(-2ll)) == (unsigned long long)id) {
-                vals[s] = INT32_MAX;   // free slots hold INT32_MAX so a later claim can atomicMin
-                atomicAdd(removed, 1);
-            }
-            return;
-        }
-        s = s + 1 == cap ? 0 : s + 1;
-    }
-}
-
-__global__ void fill_i64_kernel(unsigned long long *p, int64_t n, unsigned long long v) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < n) p[g] = v;
-}
 __global__ void fill_i32_kernel(int32_t *p, int64_t n, int32_t v) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n) p[g] = v;
@@ -1574,33 +1516,11 @@ hipError_t ss_launch_bits(const int8_t *st, uint64_t *bits, int64_t w0, int64_t 
     return hipGetLastError();
 }
 
-hipError_t ss_launch_map_clear(unsigned long long *keys, int32_t *vals, int64_t cap, hipStream_t s) {
-    hipLaunchKernelGGL(fill_i64_kernel, dim3(grid_for(cap, 256)), dim3(256), 0, s, keys, cap,
-                       (unsigned long long)(-1ll));
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_for(cap, 256)), dim3(256), 0, s, vals, cap, (int32_t)INT32_MAX);
-    return hipGetLastError();
-}
-
-hipError_t ss_launch_map_insert(unsigned long long *keys, int32_t *vals, int64_t cap, const int64_t *ids,
-                                const int32_t *slots, int64_t n, int32_t *overflow, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(map_insert_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, keys, vals, cap, ids, slots, n,
-                       overflow);
-    return hipGetLastError();
-}
-
 hipError_t ss_launch_map_lookup(const unsigned long long *keys, const int32_t *vals, int64_t cap, const int64_t *ids,
                                 int32_t *out, int64_t n, int32_t *misses, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(map_lookup_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, keys, vals, cap, ids, out, n,
                        misses);
-    return hipGetLastError();
-}
-
-hipError_t ss_launch_map_remove(unsigned long long *keys, int32_t *vals, int64_t cap, const int64_t *ids, int64_t n,
-                                int32_t *removed, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(map_remove_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, keys, vals, cap, ids, n, removed);
     return hipGetLastError();
 }
 

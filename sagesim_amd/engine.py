@@ -259,6 +259,23 @@ class Engine:
         self._ck(self._L.ss_map_lookup(self._h, rep, ptr(ids), ptr(out), ids.shape[0], ctypes.byref(miss)))
         return out, miss.value
 
+    def map_resize(self, rep, new_capacity):
+        self._ck(self._L.ss_map_resize(self._h, rep, int(new_capacity)))
+
+    def map_info(self, rep):
+        """(capacity, size) of the replica's id -> slot map."""
+        cap, size = ctypes.c_int64(), ctypes.c_int64()
+        self._ck(self._L.ss_map_info(self._h, rep, ctypes.byref(cap), ctypes.byref(size)))
+        return cap.value, size.value
+
+    def map_arrays(self, rep):
+        """Copies of the map's key and value arrays (-1 empty, -2 deleted)."""
+        cap, _ = self.map_info(rep)
+        keys = np.empty(cap, np.int64)
+        vals = np.empty(cap, np.int32)
+        self._ck(self._L.ss_map_download(self._h, rep, ptr(keys), ptr(vals), cap))
+        return keys, vals
+
     def halo_bind(self, rep, ids):
         ids = np.ascontiguousarray(ids, np.int64)
         miss = ctypes.c_int64()
