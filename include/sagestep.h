@@ -174,6 +174,15 @@ int ss_halo_unpack_ids(ss_ctx *ctx, int32_t rep, const int64_t *ids, const int8_
                        int64_t *n_missing);
 /* out[h] = current state of slot slots[h] (the send side, gpu_kernels.py:852-909) */
 int ss_halo_pack(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n, int8_t *out);
+/* ss_halo_pack with no device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, per-tick
+ * exchange loops that must not call hipMalloc).  Its staging is a fixed
+ * device array per GPU (SS_NOALLOC_MAX_RECORDS records; n beyond that is
+ * refused with SS_ERR_INVALID), held until the process ends.  Calls that
+ * overlap on one GPU (several threads or contexts on the same device) race
+ * on that array. */
+#define SS_NOALLOC_MAX_RECORDS (1 << 20)
+int ss_halo_pack_noalloc(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n, int8_t *out);
 /* Run `ticks` ticks; before tick j the ghosts are refreshed from
  * vals[j * n_bound ...] (bound order), copied on a second stream while the
  * previous tick computes: straight from vals when it lies in memory from

@@ -96,6 +96,7 @@ def step_lib():
     _proto(L, "ss_halo_unpack", I, _P, _c_i32, _P, _c_i64)
     _proto(L, "ss_halo_unpack_ids", I, _P, _c_i32, _P, _P, _c_i64, _P)
     _proto(L, "ss_halo_pack", I, _P, _c_i32, _P, _c_i64, _P)
+    _proto(L, "ss_halo_pack_noalloc", I, _P, _c_i32, _P, _c_i64, _P)
     _proto(L, "ss_simulate_halo", I, _P, _c_i32, _c_i64, _P)
     _proto(L, "ss_halo_unpack_2bit", I, _P, _c_i32, _P, _c_i64)
     _proto(L, "ss_halo_pack_2bit", I, _P, _c_i32, _P, _c_i64, _P)
