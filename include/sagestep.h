@@ -111,6 +111,16 @@ int ss_state_download(ss_ctx *ctx, int32_t rep, int8_t *out, int64_t n);
 int ss_state_upload(ss_ctx *ctx, int32_t rep, const int8_t *in, int64_t n);
 int ss_history_download(ss_ctx *ctx, int32_t rep, int32_t *out, int64_t n_rows);
 
+/* ---- checkpoint: replaces the pickle of Model.save / Model.load
+ *      (model.py:1289-1310) with the replica's device state, bit for bit ----
+ * The blob holds the tick, tick limit, seed, compartment model, the live
+ * state buffer and the history rows, with the shape, CSR hash and RNG-key hash
+ * of the replica they belong to.  buf NULL: *size = bytes needed.  Restore
+ * into a replica of the same network (any context) continues the run exactly
+ * as the saved one would; *tick gets the restored tick. */
+int ss_replica_save(ss_ctx *ctx, int32_t rep, void *buf, int64_t cap, int64_t *size);
+int ss_replica_restore(ss_ctx *ctx, int32_t rep, const void *buf, int64_t size, int64_t *tick);
+
 /* ---- run: replaces Model.simulate -> worker_coroutine -> stepfunc launch
  *      (model.py:961-998, :1314-1640, kernel template :2421-2454) ----
  * Advances every live replica by `ticks` ticks (each stops at its own
@@ -245,9 +255,10 @@ int ss_timer_stop(ss_ctx *ctx, float *ms);
  *      write-back of the fused kernel (model.py:2368-2408) ----
  * A program holds n agents (sorted by breed), their CSR, float32 agent ids,
  * logical ids and breed indices (model.py:1085-1096), float64 globals, up to
- * 48 properties (element size 1/2/4/8 bytes, `width` elements per agent,
- * double-buffered or not) and the kernels compiled from `source`, which
- * includes "sagesim_rt.h" (the device runtime this library embeds). */
+ * 64 device arrays (properties, breed-local arrays and their index maps;
+ * element size 1/2/4/8 bytes, rows x width elements, double-buffered or not),
+ * up to 16 kernel-extra buffers, and the kernels compiled from `source`,
+ * which includes "sagesim_rt.h" (the device runtime this library embeds). */
 typedef struct ss_prog ss_prog;
 int ss_prog_create(ss_ctx *ctx, int64_t n_agents, ss_prog **out);
 int ss_prog_destroy(ss_prog *prog);
@@ -257,22 +268,50 @@ int ss_prog_set_csr(ss_prog *prog, const int64_t *row_ptr, const int32_t *col_id
  * plus the logical ids as exact int64 */
 int ss_prog_set_ids(ss_prog *prog, const float *agent_ids, const float *logical_ids, const int64_t *logical_ids64,
                     const float *breeds);
+/* a property: n rows (GPUBufferManager.allocate_property_tensors and
+ * allocate_write_buffers, gpu_kernels.py:306-332) */
 int ss_prog_add_prop(ss_prog *prog, int32_t elem_bytes, int64_t width, int32_t double_buffered, const void *init,
                      int32_t *prop_id);
+/* any device array of `rows` rows: a breed-local array (model.py:1215-1270,
+ * rows = agents of the breed) or an index map (rows = n); init NULL zero-fills */
+int ss_prog_add_array(ss_prog *prog, int32_t elem_bytes, int64_t rows, int64_t width, int32_t double_buffered,
+                      const void *init, int32_t *array_id);
 int ss_prog_prop_upload(ss_prog *prog, int32_t prop_id, const void *data);
 int ss_prog_prop_download(ss_prog *prog, int32_t prop_id, void *out);
 int ss_prog_set_globals(ss_prog *prog, const double *values, int64_t count);
 int ss_prog_set_seed(ss_prog *prog, int64_t seed);
+/* kernel-extra buffers (the extra kernel arguments of _prepare_kernel_extras,
+ * model.py:892-921, :1501, :1526): slot `slot` holds `bytes` bytes, visible to
+ * the wrapper kernels as SSProgArgs::extra[slot]; read back after a run */
+int ss_prog_set_extra(ss_prog *prog, int32_t slot, int64_t bytes, const void *data);
+int ss_prog_get_extra(ss_prog *prog, int32_t slot, int64_t bytes, void *out);
 /* Compile `source` for gfx950 and load the kernels named in `kernel_names`
- * ('\n'-separated); the hipRTC log is copied into `log` (may be NULL). */
+ * ('\n'-separated); the hipRTC log is copied into `log` (may be NULL).  Code
+ * objects are cached by a hash of the source, the runtime header and the
+ * options, in the process and in $SAGESIM_AMD_RTC_CACHE (default
+ * ~/.cache/sagesim_amd/rtc; empty: no disk cache), as the reference caches
+ * its generated module (model.py:843-865). */
 int ss_prog_compile(ss_prog *prog, const char *source, const char *kernel_names, char *log, int64_t log_cap);
+/* *hit = 0: the last compile ran hipRTC; 1: process cache; 2: disk cache */
+int ss_prog_cache_hit(ss_prog *prog, int32_t *hit);
 /* The launches of one tick, in order: kernel index, first agent, agent count.
  * Every kernel takes (const SSProgArgs*, long long tick, long long start,
- * long long count). */
+ * long long count, const int *order): thread k runs agent order[k], or
+ * start + k when order is NULL. */
 int ss_prog_set_schedule(ss_prog *prog, int32_t n, const int32_t *kernel, const int64_t *start,
                          const int64_t *count);
+/* Agent order of every launch of the schedule, concatenated (each launch's
+ * segment a permutation of its range), e.g. longest rows first so that the
+ * threads of the longest rows start first; results do not depend on it. */
+int ss_prog_set_order(ss_prog *prog, const int32_t *order, int64_t n);
 /* Ticks tick0 .. tick0 + ticks - 1: the schedule, then write-back; waits. */
 int ss_prog_simulate(ss_prog *prog, int64_t tick0, int64_t ticks);
+/* Checkpoint of the program's state (replaces the pickle of Model.save /
+ * Model.load, model.py:1289-1310): globals, seed, every array, the caller's
+ * tick, and the shape of the network it belongs to.  buf NULL: *size = bytes
+ * needed.  Restore checks the shape and CSR and returns the tick. */
+int ss_prog_save(ss_prog *prog, int64_t tick, void *buf, int64_t cap, int64_t *size);
+int ss_prog_restore(ss_prog *prog, const void *buf, int64_t size, int64_t *tick);
 
 #ifdef __cplusplus
 }

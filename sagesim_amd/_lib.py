@@ -82,6 +82,8 @@ def step_lib():
     _proto(L, "ss_state_download", I, _P, _c_i32, _P, _c_i64)
     _proto(L, "ss_state_upload", I, _P, _c_i32, _P, _c_i64)
     _proto(L, "ss_history_download", I, _P, _c_i32, _P, _c_i64)
+    _proto(L, "ss_replica_save", I, _P, _c_i32, _P, _c_i64, _P)
+    _proto(L, "ss_replica_restore", I, _P, _c_i32, _P, _c_i64, _P)
     _proto(L, "ss_simulate", I, _P, _c_i64)
     _proto(L, "ss_set_tiling", I, _P, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32)
     _proto(L, "ss_launch_shape", I, _P, _P, _P)
@@ -115,6 +117,13 @@ def step_lib():
     _proto(L, "ss_prog_set_csr", I, _P, _P, _P, _c_i64)
     _proto(L, "ss_prog_set_ids", I, _P, _P, _P, _P, _P)
     _proto(L, "ss_prog_add_prop", I, _P, _c_i32, _c_i64, _c_i32, _P, _P)
+    _proto(L, "ss_prog_add_array", I, _P, _c_i32, _c_i64, _c_i64, _c_i32, _P, _P)
+    _proto(L, "ss_prog_set_extra", I, _P, _c_i32, _c_i64, _P)
+    _proto(L, "ss_prog_get_extra", I, _P, _c_i32, _c_i64, _P)
+    _proto(L, "ss_prog_cache_hit", I, _P, _P)
+    _proto(L, "ss_prog_set_order", I, _P, _P, _c_i64)
+    _proto(L, "ss_prog_save", I, _P, _c_i64, _P, _c_i64, _P)
+    _proto(L, "ss_prog_restore", I, _P, _P, _c_i64, _P)
     _proto(L, "ss_prog_prop_upload", I, _P, _c_i32, _P)
     _proto(L, "ss_prog_prop_download", I, _P, _c_i32, _P)
     _proto(L, "ss_prog_set_globals", I, _P, _P, _c_i64)

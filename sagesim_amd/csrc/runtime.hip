@@ -96,6 +96,7 @@ int check_model(ss_ctx *ctx, const ss_model *m) {
 int apply_model(ss_ctx *ctx, ss_replica &r, const ss_model *m) {
     int rc = check_model(ctx, m);
     if (rc) return rc;
+    r.model = *m;
     std::vector<uint32_t> T = threshold_table(m->beta, r.max_deg);
     dfree(r.inf_T);
     r.inf_T = nullptr;
@@ -571,10 +572,13 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
     {
         std::vector<uint32_t> lo((size_t)std::max<int64_t>(nl, 1)), hi;
         if (need_hi) hi.resize((size_t)nl);
+        uint64_t kh = 1469598103934665603ull;
         for (int64_t i = 0; i < nl; ++i) {
             lo[i] = (uint32_t)sp->rng_key_id[i];
             if (need_hi) hi[i] = (uint32_t)((uint64_t)sp->rng_key_id[i] >> 32);
+            kh = (kh ^ (uint64_t)sp->rng_key_id[i]) * 1099511628211ull;
         }
+        r.key_hash = kh;
         if (nl) R_HIP(hipMemcpy(r.rng_lo, lo.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice));
         std::vector<uint16_t> d16((size_t)std::max<int64_t>(nl, 1));
         for (int64_t i = 0; i < nl; ++i)
@@ -593,6 +597,8 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
         R_HIP(hipStreamSynchronize(ctx->stream));
         R_HIP(hipMemcpy(h, ctx->d_stats, sizeof h, hipMemcpyDeviceToHost));
         r.symmetric = h[0] == h[2] && h[1] == h[3];
+        r.csr_hash[0] = h[0];
+        r.csr_hash[1] = h[1];
     }
     if (sp->history)
         R_HIP(hipMemcpy(r.init_history, sp->history, sizeof(int32_t) * nl * SS_HISTORY_WIDTH, hipMemcpyHostToDevice));
@@ -742,6 +748,76 @@ int ss_history_download(ss_ctx *ctx, int32_t rep, int32_t *out, int64_t n_rows) 
     SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     SS_HIP(ctx, hipMemcpy(out, r.history, sizeof(int32_t) * n_rows * SS_HISTORY_WIDTH, hipMemcpyDeviceToHost));
     return SS_OK;
+}
+
+// ---- checkpoint: replaces the pickle of Model.save / Model.load
+// (model.py:1289-1310) with the replica's state, bit for bit ----
+// int64 words: magic, version, n_local, n_ghost, nnz, csr_hash[2], key_hash,
+// tick, tick_limit, seed, has_history; then the model, the live state buffer
+// and the history rows, each padded to 8 bytes.
+static const int64_t kRepMagic = 0x53534147455252ll;   // "SSAGERR"
+static const int64_t kRepWords = 12;
+
+static int64_t rep_ckpt_bytes(const ss_replica &r) {
+    return kRepWords * 8 + (int64_t)((sizeof(ss_model) + 7) / 8 * 8) + (r.n_total + 7) / 8 * 8 +
+           (r.history ? 4 * r.n_local * SS_HISTORY_WIDTH : 0);
+}
+
+int ss_replica_save(ss_ctx *ctx, int32_t rep, void *buf, int64_t cap, int64_t *size) {
+    if (!valid_rep(ctx, rep) || !size || cap < 0 || (cap && !buf)) SS_FAIL(ctx, SS_ERR_INVALID, "bad arguments");
+    const ss_replica &r = ctx->reps[rep];
+    const int64_t need = rep_ckpt_bytes(r);
+    *size = need;
+    if (!buf) return SS_OK;
+    if (cap < need) SS_FAIL(ctx, SS_ERR_INVALID, "buffer of %lld bytes, checkpoint needs %lld", (long long)cap,
+                            (long long)need);
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::memset(buf, 0, (size_t)need);
+    int64_t *h = static_cast<int64_t *>(buf);
+    const uint64_t seed = (uint64_t)r.dev.seed_lo | ((uint64_t)r.dev.seed_hi << 32);
+    const int64_t w[kRepWords] = {kRepMagic, 1, r.n_local, r.n_ghost, r.nnz, (int64_t)r.csr_hash[0],
+                                  (int64_t)r.csr_hash[1], (int64_t)r.key_hash, r.tick,
+                                  r.tick_limit == INT64_MAX ? 0 : r.tick_limit, (int64_t)seed, r.history ? 1 : 0};
+    std::memcpy(h, w, sizeof w);
+    char *c = static_cast<char *>(buf) + kRepWords * 8;
+    std::memcpy(c, &r.model, sizeof(ss_model));
+    c += (sizeof(ss_model) + 7) / 8 * 8;
+    if (r.n_total) SS_HIP(ctx, hipMemcpy(c, r.state[r.tick & 1], (size_t)r.n_total, hipMemcpyDeviceToHost));
+    c += (r.n_total + 7) / 8 * 8;
+    if (r.history)
+        SS_HIP(ctx, hipMemcpy(c, r.history, sizeof(int32_t) * r.n_local * SS_HISTORY_WIDTH, hipMemcpyDeviceToHost));
+    return SS_OK;
+}
+
+int ss_replica_restore(ss_ctx *ctx, int32_t rep, const void *buf, int64_t size, int64_t *tick) {
+    if (!valid_rep(ctx, rep) || !buf || size < kRepWords * 8) SS_FAIL(ctx, SS_ERR_INVALID, "bad arguments");
+    ss_replica &r = ctx->reps[rep];
+    const int64_t *h = static_cast<const int64_t *>(buf);
+    if (h[0] != kRepMagic || h[1] != 1) SS_FAIL(ctx, SS_ERR_INVALID, "not a replica checkpoint");
+    if (h[2] != r.n_local || h[3] != r.n_ghost || h[4] != r.nnz || h[5] != (int64_t)r.csr_hash[0] ||
+        h[6] != (int64_t)r.csr_hash[1])
+        SS_FAIL(ctx, SS_ERR_INVALID, "checkpoint of another network (agents, ghosts or CSR entries differ)");
+    if (h[7] != (int64_t)r.key_hash) SS_FAIL(ctx, SS_ERR_INVALID, "checkpoint of other RNG key ids");
+    if ((h[11] != 0) != (r.history != nullptr)) SS_FAIL(ctx, SS_ERR_INVALID, "history present in only one of them");
+    if (size != rep_ckpt_bytes(r)) SS_FAIL(ctx, SS_ERR_INVALID, "checkpoint has %lld bytes, expected %lld",
+                                           (long long)size, (long long)rep_ckpt_bytes(r));
+    const char *c = static_cast<const char *>(buf) + kRepWords * 8;
+    ss_model m;
+    std::memcpy(&m, c, sizeof m);
+    c += (sizeof(ss_model) + 7) / 8 * 8;
+    SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = apply_model(ctx, r, &m);
+    if (rc) return rc;
+    r.dev.seed_lo = (uint32_t)(uint64_t)h[10];
+    r.dev.seed_hi = (uint32_t)((uint64_t)h[10] >> 32);
+    r.tick = h[8];
+    r.tick_limit = h[9] > 0 ? h[9] : INT64_MAX;
+    if (r.history)
+        SS_HIP(ctx, hipMemcpy(r.history, c + (r.n_total + 7) / 8 * 8, sizeof(int32_t) * r.n_local * SS_HISTORY_WIDTH,
+                              hipMemcpyHostToDevice));
+    if (tick) *tick = r.tick;
+    // both state buffers, the bitmaps and the direction state of the restored tick
+    return ss_state_upload(ctx, rep, reinterpret_cast<const int8_t *>(c), r.n_total);
 }
 
 int ss_set_tiling(ss_ctx *ctx, int32_t hub_degree, int32_t heavy_degree, int32_t apl, int32_t grid_blocks,

@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/sagestep.h"
 #include "ss_device.h"
 #include "ss_map.h"
 
@@ -103,6 +104,9 @@ struct ss_replica {
     std::vector<int64_t> heavy_prefix_len;      // rows at the front with deg >= threshold
     std::vector<int64_t> edge64;                // CSR entries before agents 0, 64, 128, ... and n_total
     ss_rep_dev dev{};                           // device descriptor mirror (pointers filled)
+    ss_model model{};                           // the compartment model last applied
+    uint64_t csr_hash[2] = {0, 0};              // multiset hash of the CSR entries (checkpoint identity)
+    uint64_t key_hash = 0;                      // hash of the RNG key ids
     size_t bytes = 0;                           // device bytes held
 };
 

@@ -15,7 +15,8 @@
 //     (math_utils.py:87-189 with the injection of model.py:1903-1953).
 #pragma once
 
-#define SS_MAXP 48
+#define SS_MAXP 64        // device arrays of a program: properties, breed-local arrays and their index maps
+#define SS_MAXX 16        // kernel-extra buffers
 
 struct SSProgArgs {
     long long n_agents;
@@ -29,6 +30,7 @@ struct SSProgArgs {
     void *prop_w[SS_MAXP];
     int seed;
     const long long *logical_ids64;   // exact ids (the float32 ids round above 2^24)
+    void *extra[SS_MAXX];             // kernel-extra buffers (model.py:892-921)
 };
 
 // Seed for draws keyed on a plain id rather than the running agent.
@@ -107,32 +109,65 @@ __device__ inline double rand_uniform_philox4x32(long long tick, const AgentInde
 }
 
 // ---- properties ----
+// A write to the running agent's own slot of a double-buffered scalar
+// property is held in an OwnSlot in registers and stored once after the step
+// function returns (the wrapper kernel does that), so a step function that
+// writes its own slot inside its neighbour loop keeps no store in that loop
+// and the loop's loads can overlap.  Writes to any other slot, and every
+// write to a no_double_buffer property or a row, go to memory at once.
+// Reads always come from the read buffer, as in the reference.
+template <class T>
+struct OwnSlot {
+    T v;
+    bool dirty;
+};
+
 template <class T>
 struct PropRef {
     const T *r;
     T *w;
+    OwnSlot<T> *own;      // non-null: this is the running agent's own slot
     __device__ operator T() const { return *r; }
-    __device__ PropRef &operator=(const PropRef &o) { *w = (T)(*o.r); return *this; }
-    template <class V> __device__ PropRef &operator=(V v) { *w = (T)v; return *this; }
-    template <class V> __device__ PropRef &operator+=(V v) { *w = (T)(*r + v); return *this; }
-    template <class V> __device__ PropRef &operator-=(V v) { *w = (T)(*r - v); return *this; }
-    template <class V> __device__ PropRef &operator*=(V v) { *w = (T)(*r * v); return *this; }
-    template <class V> __device__ PropRef &operator/=(V v) { *w = (T)(*r / v); return *this; }
+    __device__ void store(T v) const {
+        if (own) {
+            own->v = v;
+            own->dirty = true;
+        } else {
+            *w = v;
+        }
+    }
+    __device__ const PropRef &operator=(const PropRef &o) const { store((T)(*o.r)); return *this; }
+    template <class V> __device__ const PropRef &operator=(V v) const { store((T)v); return *this; }
+    template <class V> __device__ const PropRef &operator+=(V v) const { store((T)(*r + v)); return *this; }
+    template <class V> __device__ const PropRef &operator-=(V v) const { store((T)(*r - v)); return *this; }
+    template <class V> __device__ const PropRef &operator*=(V v) const { store((T)(*r * v)); return *this; }
+    template <class V> __device__ const PropRef &operator/=(V v) const { store((T)(*r / v)); return *this; }
 };
 
 template <class T>
 struct Prop {
     const T *r;
     T *w;
-    __device__ PropRef<T> operator[](long long i) const { return PropRef<T>{r + i, w + i}; }
+    long long self;       // the running agent's slot, -1 when writes are not held
+    OwnSlot<T> *own;
+    __device__ PropRef<T> operator[](long long i) const {
+        const bool mine = i == self;
+        return PropRef<T>{r + i, w + i, mine ? own : (OwnSlot<T> *)nullptr};
+    }
 };
+
+// store a held own-slot write (called by the wrapper after the step returns)
+template <class T>
+__device__ inline void ss_flush(const Prop<T> &p, const OwnSlot<T> &o) {
+    if (o.dirty) p.w[p.self] = o.v;
+}
 
 template <class T>
 struct PropRowRef {
     const T *r;
     T *w;
     int width;
-    __device__ PropRef<T> operator[](long long j) const { return PropRef<T>{r + j, w + j}; }
+    __device__ PropRef<T> operator[](long long j) const { return PropRef<T>{r + j, w + j, nullptr}; }
 };
 
 template <class T>
@@ -146,6 +181,57 @@ struct PropRows {
     __device__ PropRowRef<T> operator[](long long i) const {
         return PropRowRef<T>{r + i * (long long)width, w + i * (long long)width, width};
     }
+};
+
+// ---- breed-local arrays (model.py:620-651): one row per agent of a breed,
+// `array[idx_map[agent_index]][j]` (shape (a,) per agent) or
+// `array[idx_map[agent_index]][j][k]` (shape (a, b)); read from the read
+// buffer, written to the write buffer (the same array with no_double_buffer)
+template <class T>
+struct BlaMatRef {
+    const T *r;
+    T *w;
+    int a, b;
+    __device__ PropRowRef<T> operator[](long long j) const {
+        return PropRowRef<T>{r + j * (long long)b, w + j * (long long)b, b};
+    }
+};
+
+template <class T>
+__device__ inline int len(const BlaMatRef<T> &m) { return m.a; }
+
+template <class T>
+struct BlaRows1 {
+    const T *r;
+    T *w;
+    long long rows;
+    int a;
+    __device__ PropRowRef<T> operator[](long long row) const {
+        return PropRowRef<T>{r + row * (long long)a, w + row * (long long)a, a};
+    }
+};
+
+template <class T>
+struct BlaRows2 {
+    const T *r;
+    T *w;
+    long long rows;
+    int a, b;
+    __device__ BlaMatRef<T> operator[](long long row) const {
+        const long long o = row * (long long)a * (long long)b;
+        return BlaMatRef<T>{r + o, w + o, a, b};
+    }
+};
+
+template <class T>
+__device__ inline long long len(const BlaRows1<T> &x) { return x.rows; }
+template <class T>
+__device__ inline long long len(const BlaRows2<T> &x) { return x.rows; }
+
+// buffer index -> row of a breed-local array (-1 for agents of other breeds)
+struct IdxMap {
+    const int *p;
+    __device__ int operator[](long long i) const { return p[i]; }
 };
 
 // ---- network (CSR) ----
@@ -176,7 +262,28 @@ struct GlobalArray {
     __device__ double operator[](long long i) const { return p[i]; }
 };
 
-// ---- helpers of utils.py:37-81 ----
+// ---- helpers of utils.py:34-57 ----
+// The reference's step functions see the CSR as `neighbor_offsets` and
+// `neighbor_values` (model.py:1845-1900 renames `locations`), and its helpers
+// take those: get_num_neighbors(agent_index, neighbor_offsets),
+// get_neighbor(agent_index, i, neighbor_offsets, neighbor_values).  Here the
+// two names are module globals the library points at the program's CSR, so
+// those calls compile unchanged; the forms taking `locations` work too.
+struct NeighborOffsets {
+    const long long *p;
+    __device__ long long operator[](long long i) const { return p[i]; }
+};
+struct NeighborValues {
+    const int *p;
+    __device__ int operator[](long long i) const { return p[i]; }
+};
+__device__ NeighborOffsets neighbor_offsets;
+__device__ NeighborValues neighbor_values;
+
+__device__ inline long long get_num_neighbors(long long i, const NeighborOffsets &o) { return o[i + 1] - o[i]; }
+__device__ inline int get_neighbor(long long i, long long j, const NeighborOffsets &o, const NeighborValues &v) {
+    return v[o[i] + j];
+}
 __device__ inline int get_num_neighbors(long long i, const Locations &loc) { return loc[i].n; }
 __device__ inline int get_neighbor(long long i, long long j, const Locations &loc) { return loc[i][j]; }
 template <class P>

@@ -169,6 +169,23 @@ class Engine:
         self._ck(self._L.ss_history_download(self._h, rep, ptr(out), n_local))
         return out
 
+    # -- checkpoint --
+    def replica_save(self, rep):
+        """The replica's state as a uint8 blob (ss_replica_save)."""
+        size = ctypes.c_int64()
+        self._ck(self._L.ss_replica_save(self._h, rep, None, 0, ctypes.byref(size)))
+        blob = np.zeros(size.value, np.uint8)
+        self._ck(self._L.ss_replica_save(self._h, rep, ptr(blob), blob.shape[0], ctypes.byref(size)))
+        return blob
+
+    def replica_restore(self, rep, blob):
+        """Restore a blob of replica_save into a replica of the same network;
+        returns the restored tick."""
+        blob = np.ascontiguousarray(blob, np.uint8)
+        t = ctypes.c_int64()
+        self._ck(self._L.ss_replica_restore(self._h, rep, ptr(blob), blob.shape[0], ctypes.byref(t)))
+        return t.value
+
     # -- run --
     def set_tiling(self, heavy_degree=256, agents_per_lane=16, grid_blocks=0, use_graph=True, hub_degree=1024):
         """Launch shape: rows of degree >= hub_degree get a block, >= heavy_degree a
@@ -411,6 +428,14 @@ class Replica:
     def set_states(self, state):
         state = np.asarray(state, np.int8)
         self.engine.set_state(self.rep, np.concatenate([state[:self.n][self.perm], state[self.n:]]))
+
+    def save(self):
+        """Checkpoint blob of the replica (Engine.replica_save)."""
+        return self.engine.replica_save(self.rep)
+
+    def restore(self, blob):
+        """Restore a checkpoint of a replica of this graph; returns its tick."""
+        return self.engine.replica_restore(self.rep, blob)
 
     def history(self):
         h = self.engine.history(self.rep, self.n)

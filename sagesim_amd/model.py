@@ -13,10 +13,13 @@ written back.  When the only step function is a built-in compartment step
 (sagesim_amd.steps), setup hands the model to the hand-written fused kernel
 instead; results are the same either way.
 """
+from collections import OrderedDict
 import ctypes
 from copy import copy
+import json
 from math import nan
 import os
+import re
 
 import numpy as np
 
@@ -34,7 +37,7 @@ _DTYPES = {
 
 def build_csr_from_ragged(ragged_list):
     """Ragged neighbour lists -> (offsets, values), internal_utils.py:111-164.
-    Lists keep their order; sets are written in ascending order."""
+    Every row is written in its own iteration order (a set's, a list's)."""
     lengths = [len(r) if isinstance(r, (list, tuple, set, np.ndarray)) else 0 for r in ragged_list]
     offsets = np.zeros(len(lengths) + 1, np.int64)
     np.cumsum(lengths, out=offsets[1:])
@@ -42,7 +45,7 @@ def build_csr_from_ragged(ragged_list):
     pos = 0
     for r in ragged_list:
         if isinstance(r, set):
-            r = sorted(r)
+            r = list(r)
         n = len(r) if isinstance(r, (list, tuple, np.ndarray)) else 0
         if n:
             values[pos:pos + n] = np.asarray(r, np.int64)
@@ -83,6 +86,20 @@ class _Program:
             pass
 
 
+_IDENT = re.compile(r"[A-Za-z_]\w*")
+
+
+def _split_decl(decl):
+    """'int *spikes' -> ('int *', 'spikes'); a bare name is a float buffer."""
+    decl = decl.strip()
+    if _IDENT.fullmatch(decl):
+        return "float *", decl
+    m = re.fullmatch(r"(.*?)([A-Za-z_]\w*)\s*", decl)
+    if not m or not m.group(1).strip():
+        raise ValueError(f"extra kernel parameter {decl!r} is not 'name' or 'type name'")
+    return m.group(1).strip(), m.group(2)
+
+
 class Model:
     def __init__(self, space, threads_per_block: int = 32, step_function_file_path: str = None,
                  verbose_timing: bool = False, agent_slack_factor: float = 1.5, csr_slack_factor: float = 2.0,
@@ -102,12 +119,16 @@ class Model:
         self._prop_defaults = {"breed": 0, "locations": None}
         self._prop_dtype = {}
         self._prop_maxdims = {}
+        # neighbour visibility by property (agent.py:41-48, :295-309): decides
+        # which properties a multi-rank exchange sends; one rank sends nothing
+        self._prop_visible = OrderedDict([("breed", True), ("locations", False)])
         self._agent_values = {}          # property -> list of per-agent values (creation order)
         self._agent_breed = []
         self._global_tensors = []
         self._global_names = {}
         self._seed = int(np.random.randint(0, 2 ** 31))
         self._logical_id_map = {}
+        self._breed_local_arrays = []    # {name, breed, shape_per_agent, neighbor_visible} (model.py:620-645)
         self.tick = 0
         self._is_setup = False
         self._mode = None
@@ -118,6 +139,8 @@ class Model:
         self._globals_dirty = False
         self._cache = {}
         self._tick_timings = []
+        self._last_simulate_ticks = 0
+        self._kernel_extra_arrays = []
 
     # ---- registration ----
     @property
@@ -137,12 +160,58 @@ class Model:
         breed._breedidx = len(self._breeds)
         self._breeds.append(breed)
         for name, default in breed.properties.items():
+            visible = breed.prop2neighbor_visible.get(name, True)
             if name not in self._prop_defaults:
                 self._prop_names.append(name)
                 self._agent_values[name] = []
                 self._prop_dtype[name] = breed._prop2dtype.get(name, "float32")
                 self._prop_maxdims[name] = breed._prop2maxdims.get(name)
+                self._prop_visible[name] = visible
+            else:
+                # visible when any breed marks it visible (agent.py:299-303)
+                self._prop_visible[name] = self._prop_visible.get(name, False) or visible
             self._prop_defaults[name] = default
+
+    def set_property_neighbor_visible(self, property_name: str, visible: bool) -> None:
+        """As model.py:398-409: before setup only, for a known property."""
+        if self._is_setup:
+            raise RuntimeError("Cannot change neighbor_visible after setup()")
+        if property_name not in self._prop_visible:
+            raise ValueError(f"Unknown property: {property_name}")
+        self._prop_visible[property_name] = bool(visible)
+
+    def register_breed_local_array(self, name: str, breed: Breed, shape_per_agent: tuple,
+                                   neighbor_visible: bool = False) -> int:
+        """As model.py:620-645: a float32 array with one row per agent of
+        `breed`, each row of shape `shape_per_agent` ((a,) or (a, b)), zero at
+        setup.  Step functions take it, after the properties, as the pair
+        `array, idx_map` and reach an agent's row as
+        `array[idx_map[agent_index]]`.  It is double-buffered like a property
+        unless a step function lists its name in `no_double_buffer`."""
+        if self._is_setup:
+            raise RuntimeError("register breed-local arrays before setup()")
+        if breed not in self._breeds:
+            raise ValueError(f"breed {breed.name} is not registered")
+        if not _IDENT.fullmatch(name) or any(b["name"] == name for b in self._breed_local_arrays):
+            raise ValueError(f"breed-local array name {name!r} is not a new identifier")
+        shape = tuple(int(x) for x in shape_per_agent)
+        if len(shape) not in (1, 2) or min(shape) < 1:
+            raise ValueError("shape_per_agent must be (a,) or (a, b) with positive sizes")
+        idx = len(self._breed_local_arrays)
+        self._breed_local_arrays.append({"name": name, "breed": breed, "shape_per_agent": shape,
+                                         "neighbor_visible": bool(neighbor_visible)})
+        return idx
+
+    def get_breed_local_array(self, name: str) -> np.ndarray:
+        """As model.py:647-650: the whole array (rows of the breed's agents in
+        buffer order) as float32 numpy."""
+        if not self._is_setup or self._mode != "rtc":
+            raise RuntimeError("breed-local arrays live on the device after setup()")
+        b = next(i for i, x in enumerate(self._breed_local_arrays) if x["name"] == name)
+        aid, _, rows = self._bla_ids[b]
+        out = np.empty((rows,) + self._breed_local_arrays[b]["shape_per_agent"], np.float32)
+        self._prog.ck(self._prog._L.ss_prog_prop_download(self._prog._h, aid, ptr(out)))
+        return out
 
     def create_agent_of_breed(self, breed: Breed, add_to_space=True, rank: int = None, **kwargs) -> int:
         if breed not in self._breeds:
@@ -190,14 +259,41 @@ class Model:
     def set_agent_logical_id(self, agent_id: int, logical_id: int) -> None:
         self._logical_id_map[int(agent_id)] = int(logical_id)
 
+    # ---- hooks for subclasses (model.py:892-921) ----
+    def _get_extra_kernel_config(self) -> dict:
+        """Override to extend the wrapper kernels.  Keys:
+        'extra_kernel_params': declarations of buffers the kernels see, each
+            'name' (a float buffer) or 'type name' (e.g. 'int *spikes'); the
+            arrays _prepare_kernel_extras returns fill them, in this order;
+        'post_breed_step_code': [(code_lines, once_per_breed[, only_priority])]
+            HIP lines run right after a breed's step function (its own-slot
+            writes already stored), with `_real_idx`, `thread_local_tick`,
+            `breed_id`, `agent_index`, the globals, the extra buffers, `a<k>` /
+            `write_a<k>` (property k's read and write buffers, k the property's
+            index, breed = 0 and locations = 1) and the breed-local arrays by
+            name in scope."""
+        return {}
+
+    def _prepare_kernel_extras(self, num_local_agents, sync_ticks) -> tuple:
+        """Override to provide the extra kernel buffers of a simulate() call
+        (numpy arrays, one per 'extra_kernel_params' entry).  After the run
+        their device contents are copied back into these arrays."""
+        return ()
+
+    def _process_kernel_extras(self) -> None:
+        """Override to read the extra buffers after a simulate() call."""
+        pass
+
+    def _has_kernel_extras(self):
+        return type(self)._get_extra_kernel_config is not Model._get_extra_kernel_config
+
     # ---- property access ----
     def _buffer_index(self, agent_id):
         return int(self._inv[agent_id])
 
     def get_agent_property_value(self, id: int, property_name: str):
         if property_name == "locations":
-            loc = self._space.get_location(id)
-            return list(loc) if not isinstance(loc, set) else sorted(loc)
+            return list(self._space.get_location(id))
         if property_name == "breed":
             return self._agent_breed[id]
         if self._is_setup:
@@ -233,6 +329,34 @@ class Model:
         self._cache[name] = arr
         return arr
 
+    def _breed_named(self, breed_name):
+        for b in self._breeds:
+            if b.name == breed_name:
+                return b
+        raise KeyError(breed_name)
+
+    def get_breed_data(self, breed_name, property_name, local=False):
+        """As model.py:464-488 on one rank: the property's rows of the breed's
+        agents, in buffer order (agents sorted by breed)."""
+        if not self._is_setup:
+            raise RuntimeError("call setup() first")
+        start, count = self._breed_ranges.get(self._breed_named(breed_name)._breedidx, (0, 0))
+        if property_name not in self._agent_values:
+            raise KeyError(property_name)
+        if count == 0:
+            return np.empty((0,), np.float32)
+        return np.array(self._device_property(property_name)[start:start + count])
+
+    def get_breed_agent_ids(self, breed_name, local=False):
+        """As model.py:490-515 on one rank: the breed's agent ids in buffer
+        order, float32 as the reference stores them (model.py:1085-1087)."""
+        if not self._is_setup:
+            raise RuntimeError("call setup() first")
+        start, count = self._breed_ranges.get(self._breed_named(breed_name)._breedidx, (0, 0))
+        if count == 0:
+            return np.empty((0,), np.float32)
+        return np.asarray(self._order[start:start + count]).astype(np.float32)
+
     # ---- setup ----
     def _priority_groups(self):
         prios = sorted({p for b in self._breeds for p in b.step_funcs})
@@ -264,35 +388,51 @@ class Model:
         return out, width
 
     def _csr(self, order, inv):
+        """CSR over buffer indices.  Each agent's row lists its neighbours in
+        the order its container iterates them (a set's order, or insertion
+        order with NetworkSpace(ordered=True); model.py:1143-1148,
+        internal_utils.py:111-164).  Edges added with connect_agents_bulk
+        follow, in the order given, each undirected edge (u, v) adding v to
+        u's row and then u to v's row as connect_agents would; a neighbour
+        already in the row is not added again."""
         rows = self._space.neighbor_rows()
         n = len(order)
-        if self._space.has_bulk_edges():
-            us, vs = [], []
-            for a in range(n):
-                nb = rows[a] if a < len(rows) else ()
-                if nb:
-                    us.append(np.full(len(nb), a, np.int64))
-                    vs.append(np.fromiter(nb, np.int64, len(nb)))
-            for u, v, d in zip(self._space._bulk_u, self._space._bulk_v, self._space._bulk_directed):
-                us.append(u)
-                vs.append(v)
-                if not d:
-                    us.append(v)
-                    vs.append(u)
-            u = np.concatenate(us) if us else np.zeros(0, np.int64)
-            v = np.concatenate(vs) if vs else np.zeros(0, np.int64)
-            g = graphs.from_edges(n, inv[u], inv[v], directed=True)
-            return g.row_ptr, g.col_idx
-        ragged = []
-        for a in order:
+        src, dst = [], []
+        for b, a in enumerate(order):
             nb = rows[a] if a < len(rows) else ()
-            if isinstance(nb, set):
-                nb = sorted(nb)
-            ragged.append([int(inv[x]) for x in nb])
-        return build_csr_from_ragged(ragged)
+            if nb:
+                src.append(np.full(len(nb), b, np.int64))
+                dst.append(inv[np.fromiter(nb, np.int64, len(nb))])
+        if not self._space.has_bulk_edges():
+            counts = np.zeros(n, np.int64)
+            for b, a in enumerate(order):
+                nb = rows[a] if a < len(rows) else ()
+                counts[b] = len(nb)
+            rp = np.zeros(n + 1, np.int64)
+            np.cumsum(counts, out=rp[1:])
+            ci = np.concatenate(dst).astype(np.int32) if dst else np.zeros(0, np.int32)
+            return rp, ci
+        for u, v, d in zip(self._space._bulk_u, self._space._bulk_v, self._space._bulk_directed):
+            u, v = inv[u], inv[v]
+            if d:
+                src.append(u)
+                dst.append(v)
+            else:
+                src.append(np.stack([u, v], axis=1).ravel())
+                dst.append(np.stack([v, u], axis=1).ravel())
+        s = np.concatenate(src) if src else np.zeros(0, np.int64)
+        t = np.concatenate(dst) if dst else np.zeros(0, np.int64)
+        _, first = np.unique(s * max(n, 1) + t, return_index=True)   # first occurrence of every (row, neighbour)
+        keep = np.sort(first)
+        keep = keep[np.argsort(s[keep], kind="stable")]
+        rp = np.zeros(n + 1, np.int64)
+        np.cumsum(np.bincount(s[keep], minlength=n), out=rp[1:])
+        return rp, t[keep].astype(np.int32)
 
     def _fused_spec(self, groups):
         if not self._use_fused or len(groups) != 1 or len(groups[0]) != 1:
+            return None
+        if self._breed_local_arrays or self._has_kernel_extras():
             return None
         (bidx, fn), = groups[0].items()
         spec = getattr(fn, "fused", None)
@@ -312,6 +452,12 @@ class Model:
         inv = np.empty(n, np.int64)
         inv[order] = np.arange(n)
         self._order, self._inv = order, inv
+        bs = np.asarray(self._agent_breed, np.int64)[order]
+        self._breed_ranges = {}
+        for b in self._breeds:
+            idx = np.nonzero(bs == b._breedidx)[0]
+            if idx.size:
+                self._breed_ranges[b._breedidx] = (int(idx[0]), int(idx.size))
         self._host_props, self._prop_width = {}, {}
         for name in self._prop_names[2:]:
             self._host_props[name], self._prop_width[name] = self._property_tensor(name, order)
@@ -363,10 +509,15 @@ class Model:
     def _ctype(self, name):
         return _DTYPES[self._prop_dtype[name]][0]
 
+    def _no_double_buffer(self):
+        return set().union(*(b.no_double_buffer_props for b in self._breeds)) if self._breeds else set()
+
     def generate_source(self, groups=None):
         """HIP source of the step kernels: the breeds' step functions plus one
         `ss_prio_<p>` wrapper per priority group that calls each breed's step
-        function with the reference's argument order."""
+        function with the reference's argument order (model.py:2302-2337):
+        tick, agent_index, globals, agent_ids, breeds, locations, properties
+        in registration order, then `array, idx_map` per breed-local array."""
         if groups is None:
             _, groups = self._priority_groups()
         lines = ['#include "sagesim_rt.h"', ""]
@@ -377,6 +528,7 @@ class Model:
                     lines.append(fn.source)
                     seen.add(fn.source)
         props = self._prop_names[2:]
+        nodb = self._no_double_buffer()
         gdecl, gargs, off = [], [], 0
         for gi, arr in enumerate(self._global_tensors):
             if arr.size == 1:                                  # scalars are passed by value (model.py:2229-2234)
@@ -385,32 +537,76 @@ class Model:
                 gdecl.append(f"    const GlobalArray G{gi}{{A->globals + {off}, {arr.size}}};")
             gargs.append(f"G{gi}")
             off += arr.size
-        pdecl, pargs = [], []
+        pdecl, pargs, flush = [], [], []
         for j, name in enumerate(props):
             ct = self._ctype(name)
-            if self._prop_width[name] == 1:
-                pdecl.append(f"    const Prop<{ct}> P{j}{{(const {ct} *)A->prop_r[{j}], ({ct} *)A->prop_w[{j}]}};")
+            r, w = f"(const {ct} *)A->prop_r[{j}]", f"({ct} *)A->prop_w[{j}]"
+            if self._prop_width.get(name, 1) == 1:
+                if name in nodb:
+                    pdecl.append(f"    const Prop<{ct}> P{j}{{{r}, {w}, -1, nullptr}};")
+                else:
+                    pdecl.append(f"    OwnSlot<{ct}> O{j}{{}};")
+                    pdecl.append(f"    const Prop<{ct}> P{j}{{{r}, {w}, i, &O{j}}};")
+                    flush.append(f"ss_flush(P{j}, O{j});")
             else:
-                pdecl.append(f"    const PropRows<{ct}> P{j}{{(const {ct} *)A->prop_r[{j}], ({ct} *)A->prop_w[{j}], "
-                             f"{self._prop_width[name]}}};")
+                pdecl.append(f"    const PropRows<{ct}> P{j}{{{r}, {w}, {self._prop_width[name]}}};")
+            pdecl.append(f"    const {ct} *const a{j + 2} = {r}; {ct} *const write_a{j + 2} = {w};")
+            pdecl.append(f"    (void)a{j + 2}; (void)write_a{j + 2};")
             pargs.append(f"P{j}")
+        bdecl, bargs = [], []
+        for k, bla in enumerate(self._breed_local_arrays):
+            aid, xid, rows = self._bla_ids[k] if hasattr(self, "_bla_ids") else (0, 0, 1)
+            shape = bla["shape_per_agent"]
+            r, w = f"(const float *)A->prop_r[{aid}]", f"(float *)A->prop_w[{aid}]"
+            if len(shape) == 1:
+                bdecl.append(f"    const BlaRows1<float> B{k}{{{r}, {w}, {rows}, {shape[0]}}};")
+            else:
+                bdecl.append(f"    const BlaRows2<float> B{k}{{{r}, {w}, {rows}, {shape[0]}, {shape[1]}}};")
+            bdecl.append(f"    const IdxMap B{k}_idx{{(const int *)A->prop_r[{xid}]}};")
+            bdecl.append(f"    const auto &{bla['name']} = B{k}; const auto &{bla['name']}_idx = B{k}_idx;")
+            bdecl.append(f"    (void){bla['name']}; (void){bla['name']}_idx;")
+            bargs += [f"B{k}", f"B{k}_idx"]
+        cfg = self._get_extra_kernel_config() or {}
+        xdecl = []
+        for k, decl in enumerate(cfg.get("extra_kernel_params", [])):
+            ty, nm = _split_decl(decl)
+            xdecl.append(f"    {ty} const {nm} = ({ty})A->extra[{k}]; (void){nm};")
+        post = cfg.get("post_breed_step_code", [])
+        seen_breeds = set()
         names = []
         for p, grp in enumerate(groups):
             kname = f"ss_prio_{p}"
             names.append(kname)
             lines.append(f'extern "C" __global__ void {kname}(const SSProgArgs *__restrict__ A, long long tick, '
-                         f"long long start, long long count) {{")
+                         f"long long start, long long count, const int *__restrict__ order) {{")
             lines.append("    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;")
             lines.append("    if (k >= count) return;")
-            lines.append("    const long long i = start + k;")
+            lines.append("    const long long i = order ? (long long)order[k] : start + k;")
             lines.append("    const int breed = (int)A->breeds[i];")
             lines.append("    const AgentIndex agent_index{i, A->logical_ids[i], A->seed, A->logical_ids64[i]};")
+            lines.append("    const long long _real_idx = i, thread_local_tick = tick; const int breed_id = breed;")
+            lines.append("    (void)_real_idx; (void)thread_local_tick; (void)breed_id;")
             lines.append("    const FloatArray agent_ids{A->agent_ids}, breeds{A->breeds};")
             lines.append("    const Locations locations{A->row_ptr, A->col_idx};")
-            lines += gdecl + pdecl
+            lines += gdecl + pdecl + bdecl + xdecl
             for b, fn in grp.items():
-                args = ", ".join(["tick", "agent_index"] + gargs + ["agent_ids", "breeds", "locations"] + pargs)
-                lines.append(f"    if (breed == {b}) {{ {fn.name}({args}); return; }}")
+                args = ", ".join(["tick", "agent_index"] + gargs + ["agent_ids", "breeds", "locations"] + pargs
+                                 + bargs)
+                body = [f"{fn.name}({args});"] + flush
+                for entry in post:
+                    code_lines, once = entry[0], entry[1]
+                    only = entry[2] if len(entry) > 2 else None
+                    if only is not None and p != only:
+                        continue
+                    if once and b in seen_breeds:
+                        continue
+                    body += list(code_lines)
+                    if once:
+                        seen_breeds.add(b)
+                lines.append(f"    if (breed == {b}) {{")
+                lines += [f"        {x}" for x in body]
+                lines.append("        return;")
+                lines.append("    }")
             lines.append("}")
             lines.append("")
         return "\n".join(lines), names
@@ -428,7 +624,7 @@ class Model:
         lids = np.ascontiguousarray(logical.astype(np.float32))
         breeds = np.ascontiguousarray(np.asarray(self._agent_breed, np.float32)[order])
         prog.ck(L.ss_prog_set_ids(h, ptr(aids), ptr(lids), ptr(np.ascontiguousarray(logical)), ptr(breeds)))
-        nodb = set().union(*(b.no_double_buffer_props for b in self._breeds)) if self._breeds else set()
+        nodb = self._no_double_buffer()
         self._prop_index = {}
         for name in self._prop_names[2:]:
             arr = np.ascontiguousarray(self._host_props[name])
@@ -436,6 +632,19 @@ class Model:
             prog.ck(L.ss_prog_add_prop(h, arr.dtype.itemsize, self._prop_width[name], 0 if name in nodb else 1,
                                        ptr(arr), ctypes.byref(pid)))
             self._prop_index[name] = pid.value
+        # breed-local arrays: zeroed rows of the breed's agents, and an index
+        # map buffer index -> row, -1 for other agents (model.py:1215-1270)
+        self._bla_ids = []
+        for bla in self._breed_local_arrays:
+            start, count = self._breed_ranges.get(bla["breed"]._breedidx, (0, 0))
+            rows = max(count, 1)
+            aid, xid = ctypes.c_int32(), ctypes.c_int32()
+            prog.ck(L.ss_prog_add_array(h, 4, rows, int(np.prod(bla["shape_per_agent"])),
+                                        0 if bla["name"] in nodb else 1, None, ctypes.byref(aid)))
+            idx_map = np.full(max(n, 1), -1, np.int32)
+            idx_map[start:start + count] = np.arange(count, dtype=np.int32)
+            prog.ck(L.ss_prog_add_array(h, 4, max(n, 1), 1, 0, ptr(idx_map), ctypes.byref(xid)))
+            self._bla_ids.append((aid.value, xid.value, rows))
         self._prog = prog
         self._push_globals()
         src, names = self.generate_source(groups)
@@ -448,21 +657,27 @@ class Model:
             raise NativeError(rc, L.ss_prog_last_error(h).decode())
         self._push_seed()
         # schedule: priority group p covers the breed-sorted range of its breeds
-        bs = np.asarray(self._agent_breed, np.int64)[order]
-        kern, start, count = [], [], []
+        # (model.py:1051-1062); within it the longest rows start first
+        deg = np.diff(np.asarray(rp, np.int64))
+        kern, start, count, segs = [], [], [], []
         for p, grp in enumerate(groups):
             lo, hi = n, 0
             for b in grp:
-                idx = np.nonzero(bs == b)[0]
-                if idx.size:
-                    lo, hi = min(lo, int(idx[0])), max(hi, int(idx[-1]) + 1)
+                if b in self._breed_ranges:
+                    s0, c0 = self._breed_ranges[b]
+                    lo, hi = min(lo, s0), max(hi, s0 + c0)
+            if hi <= lo:
+                lo = hi = 0
             kern.append(p)
-            start.append(lo if hi > lo else 0)
-            count.append(hi - lo if hi > lo else 0)
+            start.append(lo)
+            count.append(hi - lo)
+            segs.append(lo + np.argsort(-deg[lo:hi], kind="stable"))
         k = np.asarray(kern, np.int32)
         s = np.asarray(start, np.int64)
         c = np.asarray(count, np.int64)
         prog.ck(L.ss_prog_set_schedule(h, len(k), ptr(k), ptr(s), ptr(c)))
+        ordr = np.ascontiguousarray(np.concatenate(segs) if segs else np.zeros(0), np.int32)
+        prog.ck(L.ss_prog_set_order(h, ptr(ordr), ordr.shape[0]))
 
     def _push_globals(self):
         if self._mode == "fused":
@@ -481,26 +696,50 @@ class Model:
             self._prog.ck(self._prog._L.ss_prog_set_seed(self._prog._h, self._seed))
 
     # ---- run ----
+    def _upload_kernel_extras(self, ticks):
+        cfg = self._get_extra_kernel_config() or {}
+        params = cfg.get("extra_kernel_params", [])
+        got = tuple(self._prepare_kernel_extras(self.num_agents, int(ticks)) or ())
+        if len(got) != len(params):
+            raise ValueError(f"_prepare_kernel_extras returned {len(got)} buffers for {len(params)} parameters")
+        self._kernel_extra_arrays = []
+        for k, a in enumerate(got):
+            arr = a if isinstance(a, np.ndarray) and a.flags.c_contiguous else np.ascontiguousarray(a)
+            self._prog.ck(self._prog._L.ss_prog_set_extra(self._prog._h, k, arr.nbytes, ptr(arr)))
+            self._kernel_extra_arrays.append((a, arr))
+
+    def _download_kernel_extras(self):
+        for k, (orig, arr) in enumerate(self._kernel_extra_arrays):
+            if arr.nbytes:
+                self._prog.ck(self._prog._L.ss_prog_get_extra(self._prog._h, k, arr.nbytes, ptr(arr)))
+            if isinstance(orig, np.ndarray) and orig is not arr and orig.flags.writeable:
+                orig[...] = arr.reshape(orig.shape)
+
     def simulate(self, ticks: int, sync_workers_every_n_ticks: int = 1) -> None:
-        """As model.py:961-998 on one rank: all ticks run on the device."""
+        """As model.py:961-998 on one rank: all ticks run on the device; with
+        verbose_timing, _tick_timings gets the call's device time."""
         if not self._is_setup:
             raise RuntimeError("call setup() before simulate()")
-        if self._globals_dirty:
-            self._push_globals()
-        for name in self._dirty_props:
-            arr = np.ascontiguousarray(self._host_props[name])
-            if self._mode == "fused":
-                if name == self._fused.state:
-                    self._replica.set_states(arr.astype(np.int8))
-            else:
-                self._prog.ck(self._prog._L.ss_prog_prop_upload(self._prog._h, self._prop_index[name], ptr(arr)))
-        self._dirty_props.clear()
+        self._last_simulate_ticks = int(ticks)
+        self._tick_timings = []
+        self._flush_dirty()
         self._cache = {}
+        extras = self._mode == "rtc" and self._has_kernel_extras()
+        if extras:
+            self._upload_kernel_extras(ticks)
+        if self._verbose_timing:
+            self._engine.timer_start()
         if self._mode == "fused":
             self._engine.simulate(int(ticks))
             self._engine.sync()
         else:
             self._prog.ck(self._prog._L.ss_prog_simulate(self._prog._h, self.tick, int(ticks)))
+        if self._verbose_timing:
+            self._tick_timings.append({"ticks": int(ticks), "gpu_kernel_ms": self._engine.timer_stop(),
+                                       "mode": self._mode})
+        if extras:
+            self._download_kernel_extras()
+            self._process_kernel_extras()
         self.tick += int(ticks)
 
     def reset(self) -> None:
@@ -522,6 +761,83 @@ class Model:
         self._is_setup = False
         self._cache = {}
         self.tick = 0
+
+    # ---- checkpoint (replaces the pickle of model.py:1289-1310) ----
+    _CKPT_VERSION = 1
+
+    def _fingerprint(self):
+        return {"version": self._CKPT_VERSION, "mode": self._mode, "num_agents": self.num_agents,
+                "properties": self._prop_names[2:], "breeds": [b.name for b in self._breeds],
+                "breed_local_arrays": [b["name"] for b in self._breed_local_arrays]}
+
+    def save(self, app=None, fpath: str = None) -> None:
+        """Checkpoint the model's device state to `fpath` (an .npz written
+        with numpy, no pickle): every property, breed-local array, the globals,
+        the seed and the tick (model.py:1289-1300 pickles the whole model
+        instead).  `app` is the model to save (the reference's argument);
+        None or omitted means this model."""
+        if isinstance(app, (str, os.PathLike)) and fpath is None:
+            app, fpath = None, app
+        app = self if app is None else app
+        if not app._is_setup:
+            raise RuntimeError("save() checkpoints device state: call setup() first")
+        app._flush_dirty()
+        if app._mode == "rtc":
+            size = ctypes.c_int64()
+            app._prog.ck(app._prog._L.ss_prog_save(app._prog._h, app.tick, None, 0, ctypes.byref(size)))
+            blob = np.zeros(size.value, np.uint8)
+            app._prog.ck(app._prog._L.ss_prog_save(app._prog._h, app.tick, ptr(blob), blob.shape[0],
+                                                   ctypes.byref(size)))
+        else:
+            blob = app._engine.replica_save(app._replica.rep)
+        g = (np.concatenate([a.ravel() for a in app._global_tensors]) if app._global_tensors else np.zeros(0))
+        with open(fpath, "wb") as f:
+            np.savez(f, state=blob, globals=np.asarray(g, np.float64),
+                     meta=np.frombuffer(json.dumps({**app._fingerprint(), "tick": app.tick,
+                                                    "seed": app._seed}).encode(), np.uint8))
+
+    def load(self, fpath: str) -> "Model":
+        """Restore a checkpoint written by save() into this model, which must
+        be built and set up the same way (same agents, properties, network and
+        step functions); returns the model.  The reference unpickles a whole
+        model instead (model.py:1302-1310); here the file holds state, not code."""
+        if not self._is_setup:
+            raise RuntimeError("load() restores device state: call setup() first")
+        with np.load(fpath, allow_pickle=False) as z:
+            meta = json.loads(bytes(z["meta"]).decode())
+            blob = np.ascontiguousarray(z["state"])
+            gl = np.asarray(z["globals"], np.float64)
+        want = self._fingerprint()
+        for key in want:
+            if meta.get(key) != want[key]:
+                raise ValueError(f"checkpoint {key} {meta.get(key)!r} does not match the model's {want[key]!r}")
+        off = 0
+        for arr in self._global_tensors:
+            arr[:] = gl[off:off + arr.size].reshape(arr.shape)
+            off += arr.size
+        self._seed = int(meta["seed"])
+        self._dirty_props.clear()
+        if self._mode == "rtc":
+            t = ctypes.c_int64()
+            self._prog.ck(self._prog._L.ss_prog_restore(self._prog._h, ptr(blob), blob.shape[0], ctypes.byref(t)))
+            self.tick = t.value
+        else:
+            self.tick = self._engine.replica_restore(self._replica.rep, blob)
+        self._globals_dirty = False
+        self._cache = {}
+        return self
+
+    def _flush_dirty(self):
+        if self._globals_dirty:
+            self._push_globals()
+        for name in list(self._dirty_props):
+            arr = np.ascontiguousarray(self._host_props[name])
+            if self._mode == "fused":
+                if name == self._fused.state:
+                    self._replica.set_states(arr.astype(np.int8))
+            else:
+                self._prog.ck(self._prog._L.ss_prog_prop_upload(self._prog._h, self._prop_index[name], ptr(arr)))
+        self._dirty_props.clear()
 
     @property
     def mode(self):

@@ -77,6 +77,107 @@ def test_reference_hierarchy_sir_two_ticks():
     assert out[0] == 3 and (out[1:11] == 3).all() and (out[11:] == 2).all()
 
 
+def _f32_id(x):
+    """The reference stores logical ids as float32 (model.py:1085-1096) and the
+    RNG keys on int() of that value (model.py:1934-1951, math_utils.py:109)."""
+    return int(np.float32(float(x)))
+
+
+def _brute_ref_sir(rows, logical_ids, state, seed, tick0, ticks, p_inf, p_rec, le=False):
+    """tests/test_simulation.py:24-43 with every tick's transitions enumerated
+    from the big-int Philox words and compared as exact rationals; returns
+    the state after every tick.  le=True compares with <= (a planted mistake)."""
+    st = [int(x) for x in state]
+    out = []
+    below = (lambda a, b: a <= b) if le else (lambda a, b: a < b)
+    for t in range(tick0, tick0 + ticks):
+        nxt = list(st)
+        for r, s in enumerate(st):
+            word = P.rand_uniform_philox_words(seed, t, _f32_id(logical_ids[r]), 1)
+            u = Fraction(word % 2 ** 23, 2 ** 23)
+            if s == 2 and below(u, Fraction(p_rec)):
+                nxt[r] = 3
+            elif s == 1 and below(u, Fraction(p_inf)) and any(st[j] == 2 for j in rows[r]):
+                nxt[r] = 2
+        st = nxt
+        out.append(list(st))
+    return out
+
+
+def _word_u(seed, t, lid):
+    return (P.rand_uniform_philox_words(seed, t, _f32_id(lid), 1) % 2 ** 23) / 2 ** 23
+
+
+def _fractional_case(case):
+    """A graph of at most 8 agents, permuted logical ids (some above 2^24),
+    and p_infection / p_recovery set exactly to uniforms that a susceptible
+    and an infectious agent draw at the first tick."""
+    rng = stdlib_random.Random(100 + case)
+    n = rng.randint(4, 8)
+    edges = {(i, j) for i in range(n) for j in range(n) if i != j and rng.random() < 0.45}
+    rows = [[j for (i, j) in sorted(edges) if i == r] for r in range(n)]
+    base = rng.choice([0, 2 ** 24 - 3, 2 ** 24 + 1, 2 ** 31 + 7, 2 ** 33])
+    lids = [base + k for k in rng.sample(range(3 * n), n)]
+    seed = rng.randrange(2 ** 31)
+    state = [rng.choice([1, 1, 2, 3]) for _ in range(n)]
+    if 2 not in state:
+        state[0] = 2
+    tick0 = rng.randrange(0, 5000)
+    s_agents = [r for r in range(n) if state[r] == 1 and any(state[j] == 2 for j in rows[r])]
+    s_agents = s_agents or [r for r in range(n) if state[r] == 1] or [0]
+    i_agent = [r for r in range(n) if state[r] == 2][0]
+    p_inf = _word_u(seed, tick0, lids[s_agents[0]]) or 0.5
+    p_rec = _word_u(seed, tick0, lids[i_agent]) or 0.5
+    return rows, lids, state, seed, tick0, p_inf, p_rec, i_agent
+
+
+N_FRACTIONAL = 8
+
+
+@pytest.mark.parametrize("case", range(N_FRACTIONAL))
+def test_reference_step_brute_force_at_fractional_p(case):
+    """or_ref_sir_run at 0 < p < 1 against the brute force, every tick."""
+    rows, lids, state, seed, tick0, p_inf, p_rec, i_agent = _fractional_case(case)
+    n, ticks = len(rows), 6
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ci = np.array([j for r in rows for j in r], np.int32)
+    want = _brute_ref_sir(rows, lids, state, seed, tick0, ticks, p_inf, p_rec)
+    lid32 = np.array([np.float32(float(x)) for x in lids], np.float32)
+    st = np.array(state, np.float32)
+    for k in range(ticks):
+        st = O.ref_sir_run(rp, ci, lid32, st, seed, 1, p_inf, p_rec, tick0=tick0 + k)
+        assert st.astype(int).tolist() == want[k], (case, k)
+    # u == p_recovery at the first tick: the agent stays infectious
+    assert want[0][i_agent] == 2
+
+
+def test_reference_step_brute_force_catches_planted_mistakes():
+    """The cases above tell the right rule from `<=` and from draws keyed on
+    the slot instead of the float32 logical id: each mistake, planted in the
+    brute force, changes at least one case's trajectory."""
+    differs_le = differs_slot = 0
+    for case in range(N_FRACTIONAL):
+        rows, lids, state, seed, tick0, p_inf, p_rec, _ = _fractional_case(case)
+        want = _brute_ref_sir(rows, lids, state, seed, tick0, 6, p_inf, p_rec)
+        differs_le += _brute_ref_sir(rows, lids, state, seed, tick0, 6, p_inf, p_rec, le=True) != want
+        differs_slot += _brute_ref_sir(rows, list(range(len(rows))), state, seed, tick0, 6, p_inf, p_rec) != want
+    assert differs_le >= 1 and differs_slot >= 1, (differs_le, differs_slot)
+
+
+def test_reference_step_float32_ids_share_draws():
+    """Ids 2^24 and 2^24 + 1 are one float32 value: both agents draw the same
+    uniform (model.py:1085-1096 stores logical ids as float32)."""
+    assert _f32_id(2 ** 24 + 1) == 2 ** 24
+    lids = np.array([np.float32(2 ** 24), np.float32(2 ** 24 + 1)], np.float32)
+    assert lids[0] == lids[1]
+    rp = np.array([0, 1, 2], np.int64)
+    ci = np.array([1, 0], np.int32)
+    for seed in range(40):
+        st = O.ref_sir_run(rp, ci, lids, np.array([2, 2], np.float32), seed, 1, 0.5, 0.5)
+        assert st[0] == st[1]
+
+
 def test_reference_seed_reproducibility():
     """tests/test_simulation.py:117-159: same seed -> same states; different seeds differ."""
     g = _chain(50)

@@ -130,7 +130,7 @@ int main() {{
     A.prop_r[0] = r.data(); A.prop_w[0] = w.data();
     blockDim.x = 1;
     for (long long t = 0; t < {ticks}; ++t) {{
-        for (long long i = 0; i < n; ++i) {{ blockIdx.x = (unsigned)i; ss_prio_0(&A, t, 0, n); }}
+        for (long long i = 0; i < n; ++i) {{ blockIdx.x = (unsigned)i; ss_prio_0(&A, t, 0, n, nullptr); }}
         r = w;                                   // write-back after the tick
         A.prop_r[0] = r.data();
     }}
