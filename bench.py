@@ -97,6 +97,10 @@ def main():
                     help="step-function leg (the reference's SIR step compiled by hipRTC): agents (0 skips)")
     ap.add_argument("--stepfunc-ticks", type=int, default=100)
     ap.add_argument("--plain", action="store_true", help="only the plain 10M-agent leg")
+    ap.add_argument("--legs", default="partition,stepfunc,ensemble,serving",
+                    help="legs to run after the plain 10M-agent leg (comma-separated)")
+    ap.add_argument("--skip-plain", action="store_true",
+                    help="run only the --legs (a development run: the line then has no headline value)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -105,6 +109,14 @@ def main():
         sys.exit(2)
 
     from sagesim_amd import Engine, Replica, workloads
+    legs = {x.strip() for x in args.legs.split(",") if x.strip()}
+    if args.skip_plain:
+        eng = Engine(0)
+        out = {"development_run": "legs only, no headline value", "legs": sorted(legs)}
+        _run_legs(eng, args, legs, out)
+        print(json.dumps(out), flush=True)
+        eng.close()
+        return
 
     t0 = time.perf_counter()
     w = workloads.ba_sir(n=args.n, m=args.m, ticks=args.ticks)
@@ -284,16 +296,21 @@ def main():
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
     rep.remove()
-    if not args.plain and args.partition_ticks > 0:
-        out["partition"] = partition_leg(eng, args)
-    if not args.plain and args.stepfunc_n > 0:
-        out["stepfunc"] = stepfunc_leg(args)
-    if not args.plain and args.ensemble_replicas > 0:
-        out["ensemble"] = ensemble_leg(eng, args)
-    if not args.plain and args.serve_seconds > 0:
-        out["serving"] = serving_leg(eng, args)
+    if not args.plain:
+        _run_legs(eng, args, legs, out)
     print(json.dumps(out), flush=True)
     eng.close()
+
+
+def _run_legs(eng, args, legs, out):
+    if "partition" in legs and args.partition_ticks > 0:
+        out["partition"] = partition_leg(eng, args)
+    if "stepfunc" in legs and args.stepfunc_n > 0:
+        out["stepfunc"] = stepfunc_leg(args)
+    if "ensemble" in legs and args.ensemble_replicas > 0:
+        out["ensemble"] = ensemble_leg(eng, args)
+    if "serving" in legs and args.serve_seconds > 0:
+        out["serving"] = serving_leg(eng, args)
 
 
 def partition_leg(eng, args):

@@ -81,8 +81,20 @@ uint64_t fnv1a(uint64_t h, const void *p, size_t n) {
     return h;
 }
 
-const char *kRtcOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++20", "-ffp-contract=off",
-                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+// Loops of unknown trip count (a step function's walk over its neighbour row)
+// are unrolled at run time, so one thread has several neighbour loads in
+// flight instead of one round trip per neighbour; SAGESIM_AMD_RTC_UNROLL
+// sets the count (1 disables it).
+std::vector<std::string> rtc_options() {
+    std::vector<std::string> o = {"--offload-arch=gfx950", "-O3", "-std=c++20", "-ffp-contract=off",
+                                  "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    int unroll = 16;
+    if (const char *u = std::getenv("SAGESIM_AMD_RTC_UNROLL")) unroll = std::atoi(u);
+    if (unroll > 1) {
+        o.insert(o.end(), {"-mllvm", "-unroll-runtime", "-mllvm", "-unroll-count=" + std::to_string(unroll)});
+    }
+    return o;
+}
 
 // code objects by hash of (options, runtime header, source), for the process
 std::mutex g_cache_mu;
@@ -432,8 +444,11 @@ int ss_prog_get_extra(ss_prog *p, int32_t slot, int64_t bytes, void *out) {
 int ss_prog_compile(ss_prog *p, const char *source, const char *kernel_names, char *log, int64_t log_cap) {
     if (!p || !source || !kernel_names) P_FAIL(p, SS_ERR_INVALID, "NULL argument");
     if (log && log_cap > 0) log[0] = 0;
+    const std::vector<std::string> opts_s = rtc_options();
+    std::vector<const char *> opts;
+    for (const auto &o : opts_s) opts.push_back(o.c_str());
     uint64_t key = 1469598103934665603ull;
-    for (const char *o : kRtcOpts) key = fnv1a(key, o, std::strlen(o) + 1);
+    for (const auto &o : opts_s) key = fnv1a(key, o.c_str(), o.size() + 1);
     key = fnv1a(key, kStepRtHeader, std::strlen(kStepRtHeader));
     key = fnv1a(key, source, std::strlen(source));
     char key_hex[32];
@@ -457,7 +472,7 @@ int ss_prog_compile(ss_prog *p, const char *source, const char *kernel_names, ch
         if (hiprtcCreateProgram(&prog, source, "step_functions.hip", 1, hdrs, names) != HIPRTC_SUCCESS)
             P_FAIL(p, SS_ERR_RTC, "hiprtcCreateProgram failed");
         // packed-FP32 instructions stay out of the generated code object
-        hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof kRtcOpts / sizeof kRtcOpts[0]), kRtcOpts);
+        hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
         size_t ls = 0;
         hiprtcGetProgramLogSize(prog, &ls);
         std::string lg(ls, '\0');
