@@ -93,6 +93,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=50.0, help="serving leg: arrivals per second")
     ap.add_argument("--partition-ticks", type=int, default=150,
                     help="partition leg (one SBM part with ghosts and a halo feed): ticks (0 skips)")
+    ap.add_argument("--partition-deg-out", default="2.0,0.17",
+                    help="inter-block mean degrees of the partition leg: the stated SBM and the variant "
+                         "calibrated to ~0.6M ghosts")
     ap.add_argument("--stepfunc-n", type=int, default=1_000_000,
                     help="step-function leg (the reference's SIR step compiled by hipRTC): agents (0 skips)")
     ap.add_argument("--stepfunc-ticks", type=int, default=100)
@@ -304,7 +307,8 @@ def main():
 
 def _run_legs(eng, args, legs, out):
     if "partition" in legs and args.partition_ticks > 0:
-        out["partition"] = partition_leg(eng, args)
+        for k, d in enumerate(float(x) for x in args.partition_deg_out.split(",") if x.strip()):
+            out["partition" if k == 0 else f"partition_deg_out_{d:g}"] = partition_leg(eng, args, d)
     if "stepfunc" in legs and args.stepfunc_n > 0:
         out["stepfunc"] = stepfunc_leg(args)
     if "ensemble" in legs and args.ensemble_replicas > 0:
@@ -313,7 +317,7 @@ def _run_legs(eng, args, legs, out):
         out["serving"] = serving_leg(eng, args)
 
 
-def partition_leg(eng, args):
+def partition_leg(eng, args, deg_out=2.0):
     """One partition's share of a 32M-agent stochastic block model (64 blocks,
     mean degree 12 inside a block and 2 across, 8 parts, part 0): local rows,
     ghost slots whose scrambled int64 ids are resolved through the GPU hash map
@@ -326,7 +330,7 @@ def partition_leg(eng, args):
     against the oracle (states and history)."""
     from sagesim_amd import Replica, pack_2bit, sir_model, workloads
     t0 = time.perf_counter()
-    pw = workloads.sbm_partition(ticks=args.partition_ticks)
+    pw = workloads.sbm_partition(ticks=args.partition_ticks, deg_out=deg_out)
     g = pw.graph
     n_ghost = g.n_cols - g.n
     t_gen = time.perf_counter() - t0
@@ -381,7 +385,7 @@ def partition_leg(eng, args):
     rate = g.n * pw.ticks / t_run
     log(f"partition: {pw.ticks} ticks in {t_run * 1e3:.1f} ms ({rate:.3g} local agent-ticks/s), "
         f"halo {(n_halo + 3) // 4 / 1e6:.2f} MB per tick (2-bit), oracle window exact {exact}")
-    return {"workload": "SIR on SBM part 0 of 8 (64 blocks x 500k agents, mean degree 12 in, 2 out), "
+    return {"workload": f"SIR on SBM part 0 of 8 (64 blocks x 500k agents, mean degree 12 in, {deg_out:g} out), "
                         "ghost ids through the hash map, host halo every tick as 2-bit codes, history 32",
             "local_agents": int(g.n), "ghosts": int(n_ghost), "nnz": int(g.nnz), "ticks": pw.ticks,
             "halo_ids_missing": int(miss), "seconds": round(t_run, 4), "ms_per_tick": round(t_run * 1e3 / pw.ticks, 4),

@@ -172,6 +172,23 @@ int ss_map_info(ss_ctx *ctx, int32_t rep, int64_t *capacity, int64_t *size);
 int ss_map_download(ss_ctx *ctx, int32_t rep, int64_t *keys, int32_t *vals, int64_t capacity);
 /* out[i] = slot of ids[i] or -1 (GPUHashMap.lookup_cpu, gpu_kernels.py:208-215) */
 int ss_map_lookup(ss_ctx *ctx, int32_t rep, const int64_t *ids, int32_t *out, int64_t n, int64_t *n_missing);
+/* discover_ghost_topology (gpu_kernels.py:35-88) and the receive side of
+ * build_communication_maps (gpu_kernels.py:521-850) on the GPU.  nbr_ids are
+ * the local agents' neighbour ids (the flattened CSR values; negative entries
+ * are padding), owner[id] the rank that owns agent id (-1: none; ids at or
+ * beyond n_owner are ignored, as the reference's dense rank table ignores
+ * them).  Ghosts are the distinct ids whose owner is neither -1 nor my_rank:
+ * ghost_ids gets them in ascending order (np.unique), which is also their
+ * ghost-slot order (model.py:1069-1073); for every rank r,
+ * recv_index[recv_offsets[r] .. recv_offsets[r + 1]) lists the positions in
+ * ghost_ids of the ghosts r owns, ascending (the per-rank np.unique of
+ * gpu_kernels.py:610-617).  recv_offsets has n_ranks + 1 entries.  With
+ * ghost_ids NULL only *n_ghost is set; an owner rank >= n_ranks is
+ * SS_ERR_RANGE.  The send side is ss_map_lookup of the ids a peer requests in
+ * a map built over the local (and ghost) agents (model.py:1112-1115). */
+int ss_ghost_discover(ss_ctx *ctx, const int64_t *nbr_ids, int64_t nnz, const int32_t *owner, int64_t n_owner,
+                      int32_t my_rank, int32_t n_ranks, int64_t *ghost_ids, int32_t *recv_offsets,
+                      int32_t *recv_index, int64_t cap, int64_t *n_ghost);
 /* Fix the record order of the halo buffer: record h refreshes the ghost with
  * id ids[h] (build_communication_maps, gpu_kernels.py:521-850, resolves the
  * recv index lists once). */

@@ -95,6 +95,7 @@ def step_lib():
     _proto(L, "ss_map_info", I, _P, _c_i32, _P, _P)
     _proto(L, "ss_map_download", I, _P, _c_i32, _P, _P, _c_i64)
     _proto(L, "ss_halo_bind", I, _P, _c_i32, _P, _c_i64, _P)
+    _proto(L, "ss_ghost_discover", I, _P, _P, _c_i64, _P, _c_i64, _c_i32, _c_i32, _P, _P, _P, _c_i64, _P)
     _proto(L, "ss_halo_unpack", I, _P, _c_i32, _P, _c_i64)
     _proto(L, "ss_halo_unpack_ids", I, _P, _c_i32, _P, _P, _c_i64, _P)
     _proto(L, "ss_halo_pack", I, _P, _c_i32, _P, _c_i64, _P)

@@ -293,6 +293,23 @@ class Engine:
         self._ck(self._L.ss_map_download(self._h, rep, ptr(keys), ptr(vals), cap))
         return keys, vals
 
+    def ghost_discover(self, nbr_ids, owner, my_rank, n_ranks):
+        """Ghost ids (ascending) and, per owner rank r, the positions of r's
+        ghosts in them: ghost_ids[index[offsets[r]:offsets[r + 1]]] are the ids
+        this rank needs from r (ss_ghost_discover)."""
+        nbr = np.ascontiguousarray(nbr_ids, np.int64)
+        own = np.ascontiguousarray(owner, np.int32)
+        n = ctypes.c_int64()
+        self._ck(self._L.ss_ghost_discover(self._h, ptr(nbr), nbr.shape[0], ptr(own), own.shape[0], int(my_rank),
+                                           int(n_ranks), None, None, None, 0, ctypes.byref(n)))
+        ids = np.empty(n.value, np.int64)
+        offsets = np.empty(int(n_ranks) + 1, np.int32)
+        index = np.empty(n.value, np.int32)
+        self._ck(self._L.ss_ghost_discover(self._h, ptr(nbr), nbr.shape[0], ptr(own), own.shape[0], int(my_rank),
+                                           int(n_ranks), ptr(ids), ptr(offsets), ptr(index), ids.shape[0],
+                                           ctypes.byref(n)))
+        return ids, offsets, index
+
     def halo_bind(self, rep, ids):
         ids = np.ascontiguousarray(ids, np.int64)
         miss = ctypes.c_int64()

@@ -96,3 +96,83 @@ def test_ensemble_scenarios_distribution():
     assert {s.graph_param for s in sc} <= set(range(2, 9))
     assert all(0.01 <= s.beta <= 0.08 and 0.05 <= s.gamma <= 0.2 for s in sc)
     assert len({s.seed for s in sc}) == 256
+
+
+class SimClock:
+    """A simulated clock: every reading moves it on by 1 ms (so an idle loop
+    reaches the next arrival), and the fake engine adds its run time."""
+
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        self.t += 1e-3
+        return self.t
+
+
+class TimedEngine(FakeEngine):
+    def __init__(self, clock, step):
+        super().__init__()
+        self.clock, self.step = clock, step
+
+    def simulate(self, k):
+        super().simulate(k)
+        self.clock.t += self.step
+
+
+def test_cancel_in_every_stage_and_others_unchanged():
+    """Requests cancelled before arrival, while queued and while on the device
+    leave with no result, free their bytes, and every other request finishes
+    with the result it gets alone."""
+    arrivals = _small_requests(12)
+    arrivals = [(0.0, sc) for _, sc in arrivals[:4]] + [(5.0 + 0.001 * i, sc) for i, (_, sc) in enumerate(arrivals[4:])]
+    clock = SimClock()
+    eng = TimedEngine(clock, 0.05)
+    fps = [footprint(sc.build_graph()) for _, sc in arrivals]
+    budget = int(max(fps) * 2.2)                     # about two in flight: others wait in the queue
+    srv = Server(eng, lambda req: FakeReplica(eng, req), budget_bytes=budget, chunk_ticks=5, builders=2,
+                 clock=clock)
+    srv.cancel(9)                                    # before it arrives
+    orig = eng.simulate
+
+    def simulate_and_cancel(k):
+        orig(k)
+        if eng.calls == 2:
+            for r in eng.live[:1]:                   # the first one on the device
+                srv.cancel(r.req.rid)
+            srv.cancel(3)                            # waiting for memory or in flight
+    eng.simulate = simulate_and_cancel
+    stats, done = srv.run(arrivals)
+    srv.close()
+    by = {r.rid: r for r in done}
+    assert len(by) == len(arrivals)
+    cancelled = {rid for rid, r in by.items() if r.outcome == "cancelled"}
+    assert 9 in cancelled and 3 in cancelled and len(cancelled) == stats.cancelled >= 3
+    assert stats.completed == len(arrivals) - stats.cancelled and stats.expired == 0
+    for rid, r in by.items():
+        if r.outcome == "cancelled":
+            assert r.result is None and r.replica is None
+        else:
+            assert r.outcome == "done" and (r.result == rid % 3).all()
+    assert eng.live == []                            # every replica removed
+    assert stats.max_bytes <= budget
+
+
+def test_deadline_expires_slow_requests():
+    """With a deadline shorter than a long request's run, that request expires
+    on the device; short requests still finish, and a per-request deadline in
+    the arrival tuple overrides the server's."""
+    base = _small_requests(6)
+    long_sc = Scenario(n=1000, graph_kind="ws", graph_param=10, beta=0.05, gamma=0.1, sigma=0.0, seed=4, ticks=400)
+    arrivals = [(0.0, sc) for _, sc in base] + [(0.0, long_sc), (0.0, long_sc, 1e9)]
+    clock = SimClock()
+    eng = TimedEngine(clock, 0.05)                   # 5 ticks per 0.05 s: 400 ticks take 4 s
+    srv = Server(eng, lambda req: FakeReplica(eng, req), budget_bytes=1 << 40, chunk_ticks=5, builders=2,
+                 clock=clock, deadline_s=2.5)
+    stats, done = srv.run(arrivals)
+    srv.close()
+    by = {r.rid: r for r in done}
+    assert by[6].outcome == "expired" and by[6].result is None
+    assert by[7].outcome == "done"                   # its own deadline is far off
+    assert all(by[i].outcome == "done" for i in range(6))
+    assert stats.expired == 1 and stats.completed == 7 and eng.live == []
