@@ -53,17 +53,30 @@ def cpu_model():
     return "unknown"
 
 
-def read_traffic(path):
+def read_traffic(path, shape):
     """HBM bytes per tick-kernel launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
-    recorded as 'traffic_bytes_per_launch: <float>' in a committed profile summary."""
+    recorded as 'traffic_bytes_per_launch: <float>' in a committed profile summary together with
+    the launch it was measured on ('shape: hub=.. heavy=.. apl=.. grid=.. kernel=..').  Returns
+    (bytes or None, reason): None when the file is missing or was measured on another shape."""
+    fields, val = {}, None
     try:
         with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                    val = float(line.split(":", 1)[1])
+                elif line.startswith("shape:"):
+                    for kv in line.split(":", 1)[1].split():
+                        k, _, v = kv.partition("=")
+                        fields[k] = v
     except OSError:
-        pass
-    return None
+        return None, f"no traffic profile at {os.path.relpath(path, ROOT)}"
+    if val is None:
+        return None, "traffic profile holds no traffic_bytes_per_launch"
+    want = {k: str(v) for k, v in shape.items()}
+    got = {k: fields.get(k) for k in want}
+    if got != want:
+        return None, f"traffic profile measured on {got}, this run is {want}"
+    return val, "rocprofv3 FETCH_SIZE + WRITE_SIZE of this launch shape, calibrated"
 
 
 def main():
@@ -78,6 +91,8 @@ def main():
     ap.add_argument("--heavy", type=int, default=256, help="degree from which a row gets a whole wavefront")
     ap.add_argument("--apl", type=int, default=16, help="agents per lane of the other rows")
     ap.add_argument("--no-autotune", action="store_true", help="use --hub/--heavy/--apl as given")
+    ap.add_argument("--gather-mib", type=int, default=2048,
+                    help="span of the random-gather probe (0 skips it): the known-count gather for the counters")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the plain-copy reference run (achievable HBM bandwidth; 0 skips it)")
     ap.add_argument("--grid", type=int, default=0, help="blocks of the persistent grid (0: from occupancy)")
@@ -91,6 +106,8 @@ def main():
     ap.add_argument("--ensemble-ticks", type=int, default=300)
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving leg: seconds of arrivals (0 skips)")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="serving leg: arrivals per second")
+    ap.add_argument("--serve-capacity", type=int, default=200,
+                    help="serving leg: requests released at once to measure capacity (0 skips)")
     ap.add_argument("--partition-ticks", type=int, default=150,
                     help="partition leg (one SBM part with ghosts and a halo feed): ticks (0 skips)")
     ap.add_argument("--partition-deg-out", default="2.0,0.17",
@@ -134,6 +151,14 @@ def main():
         ms = eng.copy_probe(nb, 5)
         copy_gbs = 2 * nb / (ms * 1e-3) / 1e9
         log(f"plain copy of {args.copy_mib} MiB: {ms:.3f} ms, {copy_gbs:.0f} GB/s read+write")
+    gather = None
+    if args.gather_mib > 0:
+        n_g = 1 << 26
+        ms = eng.gather_probe(args.gather_mib << 20, n_g, 3)
+        gather = {"span_MiB": args.gather_mib, "gathers": n_g, "ms": round(ms, 4),
+                  "gathers_per_s": n_g / (ms * 1e-3)}
+        log(f"random gather probe: {n_g} 4-byte reads over {args.gather_mib} MiB in {ms:.3f} ms "
+            f"({n_g / (ms * 1e-3) / 1e9:.1f} G gathers/s)")
     eng.set_tiling(args.heavy, args.apl, args.grid, True, hub_degree=args.hub)
     eng.set_push(args.push)
     t0 = time.perf_counter()
@@ -206,10 +231,16 @@ def main():
         per_launch = b_state / args.ticks
         achieved = per_launch / (launch_us * 1e-6) / 1e9
         full_scan = 2 * g.n + 8 * (g.n + 1) + 5 * g.nnz
-        traffic = read_traffic(args.traffic)
+        shape = {"hub": args.hub, "heavy": args.heavy, "apl": args.apl, "grid": grid}
+        traffic, traffic_src = read_traffic(args.traffic, shape)
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": None if traffic is None else round(traffic, 1),
+                    "traffic_source": traffic_src,
+                    "achieved_counter_GBs": None if traffic is None else
+                    round(traffic / (launch_us * 1e-6) / 1e9, 2),
+                    "frac_counter": None if traffic is None else
+                    round(traffic / (launch_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
                     "alg_bytes_per_launch": round(per_launch, 1),
                     "byte_model": "state-aware: 2N + sum over susceptible agents of (8 + 5 degree)",
                     "touched_bytes_per_launch": round(touched / args.ticks, 1),
@@ -218,6 +249,7 @@ def main():
                     "full_scan_equivalent_GBs": round(full_scan / (launch_us * 1e-6) / 1e9, 2),
                     "launch_us": round(launch_us, 2), "push_ticks": push_ticks,
                     "plain_copy_GBs": None if copy_gbs is None else round(copy_gbs, 1),
+                    "gather_probe": gather,
                     "kernel": f"tick_kernel<{args.apl}, false>"}
         log(f"state-aware {per_launch / 1e6:.2f} MB/launch -> {achieved:.1f} GB/s; touched "
             f"{touched / args.ticks / 1e6:.2f} MB/launch; push ticks {push_ticks}")
@@ -268,7 +300,23 @@ def main():
                   f"GPU state (bit-exact vs GPU: {checked})")
         if full is not None:
             sample += f", then the whole {args.ticks}-tick run from the initial state (final states bit-exact: {full})"
+        # one thread on one window of the epidemic's busiest stretch (BASELINE.md asks for all host
+        # threads and for 1), checked against the GPU too
+        s1 = min(args.ticks // 20, max(0, args.ticks - win))
+        rep.reset()
+        eng.simulate(s1)
+        st0 = rep.states()
+        eng.simulate(win)
+        want = rep.states()
+        c0 = time.perf_counter()
+        got, _ = O.sir_run(om, g.row_ptr, g.col_idx, ids, st0, win, tick0=s1, nthreads=1)
+        dt1 = time.perf_counter() - c0
+        one = {"value": g.n * win / dt1, "cores": 1, "seconds": round(dt1, 2),
+               "sample": f"ticks [{s1}, {s1 + win}) from the GPU state", "bitexact": bool(np.array_equal(got, want))}
+        log(f"cpu 1 thread, ticks [{s1}, {s1 + win}): {dt1:.2f}s ({g.n * win / dt1:.3g} agent-ticks/s), "
+            f"match {one['bitexact']}")
         cpu = {"value": g.n * cpu_ticks / cpu_time, "unit": "agent-ticks/s", "cores": threads, "kind": "port",
+               "one_thread": one,
                "sample": sample, "seconds": round(cpu_time, 2), "windows_bitexact": checked,
                "whole_run_bitexact": full}
 
@@ -504,7 +552,12 @@ def serving_leg(eng, args):
         sc = req.scenario
         return Replica(eng, model=sc.model(), seed=sc.seed, tick_limit=sc.ticks, data=req.data)
 
-    srv = Server(eng, make, budget_bytes=200 << 30, chunk_ticks=10, builders=max(1, host_threads() - 1))
+    def progress(now, st, inflight, ready, building, arrived):
+        log(f"serving: t={now:.0f}s arrived {arrived}, done {st.completed}, in flight {inflight}, "
+            f"queued {ready}, building {building}")
+
+    srv = Server(eng, make, budget_bytes=200 << 30, chunk_ticks=10, builders=max(1, host_threads() - 1),
+                 progress=progress, progress_every_s=30.0)
     stats, done = srv.run(arrivals)
     srv.close()
     summ = stats.summary()
@@ -520,6 +573,18 @@ def serving_leg(eng, args):
     summ["agent_ticks"] = int(sum(r.scenario.n * r.scenario.ticks for r in done if r.result is not None))
     summ["rate_per_s"] = args.serve_rate
     summ["arrival_window_s"] = args.serve_seconds
+    if args.serve_capacity > 0:
+        # capacity: every request of a sample there at once (saturating arrivals)
+        burst = [(0.0, sc) for _, sc in serving_requests(rate=50.0, duration=args.serve_capacity / 50.0, seed=4242)]
+        srv = Server(eng, make, budget_bytes=200 << 30, chunk_ticks=10, builders=max(1, host_threads() - 1))
+        cst, cdone = srv.run(burst, realtime=False)
+        srv.close()
+        cs = cst.summary()
+        summ["capacity"] = {"requests": len(burst), "completed": cs["completed"],
+                            "scenarios_per_s": round(cs["scenarios_per_s"], 3), "span_s": round(cs["span_s"], 3),
+                            "agent_ticks": int(sum(r.scenario.n * r.scenario.ticks for r in cdone
+                                                   if r.outcome == "done"))}
+        log(f"serving capacity: {summ['capacity']}")
     log(f"serving: {summ}")
     for k in ("p50_latency_s", "p99_latency_s", "max_latency_s", "scenarios_per_s", "span_s"):
         summ[k] = round(summ[k], 4)

@@ -262,6 +262,12 @@ int ss_census(ss_ctx *ctx, int32_t rep, int64_t *out);
  * average milliseconds per copy.  The known-size reference for hardware
  * traffic counters and for the achievable HBM bandwidth. */
 int ss_copy_probe(ss_ctx *ctx, int64_t bytes, int32_t repeats, float *ms);
+/* n_gathers 4-byte reads at scattered positions of a fresh buffer of
+ * span_bytes (pick it far larger than the caches), repeated `repeats` times:
+ * average milliseconds per pass.  The known-count reference of a random
+ * gather for hardware traffic counters (the tick kernel's neighbour reads are
+ * gathers, not streams). */
+int ss_gather_probe(ss_ctx *ctx, int64_t span_bytes, int64_t n_gathers, int32_t repeats, float *ms);
 /* HIP events on the context's stream around queued work */
 int ss_timer_start(ss_ctx *ctx);
 int ss_timer_stop(ss_ctx *ctx, float *ms);

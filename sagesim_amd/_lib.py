@@ -110,6 +110,7 @@ def step_lib():
     _proto(L, "ss_read_counters", I, _P, _P)
     _proto(L, "ss_census", I, _P, _c_i32, _P)
     _proto(L, "ss_copy_probe", I, _P, _c_i64, _c_i32, ctypes.POINTER(ctypes.c_float))
+    _proto(L, "ss_gather_probe", I, _P, _c_i64, _c_i64, _c_i32, ctypes.POINTER(ctypes.c_float))
     _proto(L, "ss_set_push", I, _P, ctypes.c_double)
     _proto(L, "ss_replica_push_ok", I, _P, _c_i32, _P)
     _proto(L, "ss_prog_create", I, _P, _c_i64, PP)

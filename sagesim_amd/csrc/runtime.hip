@@ -1356,6 +1356,30 @@ int ss_copy_probe(ss_ctx *ctx, int64_t bytes, int32_t repeats, float *ms) {
     return SS_OK;
 }
 
+int ss_gather_probe(ss_ctx *ctx, int64_t span_bytes, int64_t n_gathers, int32_t repeats, float *ms) {
+    if (!ctx || !ms || span_bytes < 4 || span_bytes % 4 || n_gathers < 1 || repeats < 1)
+        SS_FAIL(ctx, SS_ERR_INVALID, "bad probe arguments");
+    const int grid = ctx->num_cus * 8;
+    uint32_t *src = nullptr, *out = nullptr;
+    SS_HIP(ctx, hipMalloc((void **)&src, (size_t)span_bytes));
+    hipError_t e = hipMalloc((void **)&out, sizeof(uint32_t) * (size_t)grid * 256);
+    if (e != hipSuccess) { dfree(src); SS_HIP(ctx, e); }
+    const int64_t words = span_bytes / 4;
+    e = ss_launch_fill_u32(src, words, 0x9E3779B9u, grid, ctx->stream);
+    if (e == hipSuccess) e = ss_launch_gather_probe(src, words, n_gathers, out, grid, ctx->stream);   // warm
+    if (e == hipSuccess) e = hipEventRecord(ctx->t_start, ctx->stream);
+    for (int r = 0; r < repeats && e == hipSuccess; ++r)
+        e = ss_launch_gather_probe(src, words, n_gathers, out, grid, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->t_stop, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->t_stop);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, ctx->t_start, ctx->t_stop);
+    if (e == hipSuccess) *ms /= (float)repeats;
+    dfree(src);
+    dfree(out);
+    SS_HIP(ctx, e);
+    return SS_OK;
+}
+
 int ss_timer_start(ss_ctx *ctx) {
     if (!ctx) SS_FAIL(nullptr, SS_ERR_INVALID, "ctx is NULL");
     SS_HIP(ctx, hipEventRecord(ctx->t_start, ctx->stream));

@@ -48,6 +48,9 @@ hipError_t ss_launch_gather(const int8_t *st, const int32_t *slots, int8_t *out,
 hipError_t ss_launch_gather2(const int8_t *st, const int32_t *slots, uint8_t *out, int64_t n, hipStream_t s);
 hipError_t ss_launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s);
 hipError_t ss_launch_copy(const void *src, void *dst, int64_t bytes, int grid, hipStream_t s);
+// n scattered 4-byte reads of src[0 .. n_words); out holds grid * 256 words
+hipError_t ss_launch_gather_probe(const uint32_t *src, int64_t n_words, int64_t n, uint32_t *out, int grid,
+                                  hipStream_t s);
 hipError_t ss_launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, int grid, hipStream_t s);
 hipError_t ss_launch_check_cols(const int32_t *col, int64_t nnz, int64_t n_total, unsigned long long *first_bad,
                                 int grid, hipStream_t s);

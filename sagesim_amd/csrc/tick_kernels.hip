@@ -1420,6 +1420,20 @@ __global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ d
         dst[i] = src[i];
 }
 
+// n 4-byte reads at scattered positions of src (n_words words, far larger
+// than the caches): the known-count random-gather reference for the traffic
+// counters.  One word per thread is written so the reads stay.
+__global__ void gather_probe_kernel(const uint32_t *__restrict__ src, int64_t n_words, int64_t n,
+                                    uint32_t *__restrict__ out) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t acc = 0u;
+    for (int64_t i = g; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t h = (uint64_t)i * 0x9E3779B97F4A7C15ull;
+        acc += src[(int64_t)((h >> 17) % (uint64_t)n_words)];
+    }
+    out[g] = acc;
+}
+
 __global__ void fill_u32_kernel(uint32_t *p, int64_t n, uint32_t v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v ^ (uint32_t)i;
@@ -1574,6 +1588,12 @@ hipError_t ss_launch_stats(const int8_t *st, const int64_t *row_ptr, int64_t n_l
 
 hipError_t ss_launch_copy(const void *src, void *dst, int64_t bytes, int grid, hipStream_t s) {
     hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, s, (const uint4 *)src, (uint4 *)dst, bytes / 16);
+    return hipGetLastError();
+}
+
+hipError_t ss_launch_gather_probe(const uint32_t *src, int64_t n_words, int64_t n, uint32_t *out, int grid,
+                                  hipStream_t s) {
+    hipLaunchKernelGGL(gather_probe_kernel, dim3(grid), dim3(256), 0, s, src, n_words, n, out);
     return hipGetLastError();
 }
 

@@ -243,6 +243,12 @@ class Engine:
         self._ck(self._L.ss_copy_probe(self._h, int(nbytes), int(repeats), ctypes.byref(ms)))
         return ms.value
 
+    def gather_probe(self, span_bytes, n_gathers, repeats=3):
+        """Average ms of n_gathers scattered 4-byte reads over a buffer of span_bytes."""
+        ms = ctypes.c_float()
+        self._ck(self._L.ss_gather_probe(self._h, int(span_bytes), int(n_gathers), int(repeats), ctypes.byref(ms)))
+        return ms.value
+
     def timer_start(self):
         self._ck(self._L.ss_timer_start(self._h))
 
