@@ -55,28 +55,28 @@ def cpu_model():
 
 def read_traffic(path, shape):
     """HBM bytes per tick-kernel launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
-    recorded as 'traffic_bytes_per_launch: <float>' in a committed profile summary together with
-    the launch it was measured on ('shape: hub=.. heavy=.. apl=.. grid=.. kernel=..').  Returns
-    (bytes or None, reason): None when the file is missing or was measured on another shape."""
-    fields, val = {}, None
+    from a committed profile summary holding one record per measured launch shape: a line
+    'shape: hub=.. heavy=.. apl=.. grid=..' followed by 'traffic_bytes_per_launch: <float>' and
+    'traffic_bytes_range: <lo> <hi>'.  Returns (bytes, (lo, hi), source) for the record of this
+    run's shape, or (None, None, reason)."""
+    want = {k: str(v) for k, v in shape.items()}
+    recs, cur = [], None
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    val = float(line.split(":", 1)[1])
-                elif line.startswith("shape:"):
-                    for kv in line.split(":", 1)[1].split():
-                        k, _, v = kv.partition("=")
-                        fields[k] = v
+                if line.startswith("shape:"):
+                    cur = {"shape": dict(kv.partition("=")[::2] for kv in line.split(":", 1)[1].split())}
+                    recs.append(cur)
+                elif cur is not None and line.startswith("traffic_bytes_per_launch:"):
+                    cur["bytes"] = float(line.split(":", 1)[1])
+                elif cur is not None and line.startswith("traffic_bytes_range:"):
+                    cur["range"] = tuple(float(x) for x in line.split(":", 1)[1].split())
     except OSError:
-        return None, f"no traffic profile at {os.path.relpath(path, ROOT)}"
-    if val is None:
-        return None, "traffic profile holds no traffic_bytes_per_launch"
-    want = {k: str(v) for k, v in shape.items()}
-    got = {k: fields.get(k) for k in want}
-    if got != want:
-        return None, f"traffic profile measured on {got}, this run is {want}"
-    return val, "rocprofv3 FETCH_SIZE + WRITE_SIZE of this launch shape, calibrated"
+        return None, None, f"no traffic profile at {os.path.relpath(path, ROOT)}"
+    for r in recs:
+        if {k: r["shape"].get(k) for k in want} == want and "bytes" in r:
+            return r["bytes"], r.get("range"), "rocprofv3 FETCH_SIZE + WRITE_SIZE of this launch shape, calibrated"
+    return None, None, f"no traffic record for launch shape {want} in {os.path.relpath(path, ROOT)}"
 
 
 def main():
@@ -232,11 +232,12 @@ def main():
         achieved = per_launch / (launch_us * 1e-6) / 1e9
         full_scan = 2 * g.n + 8 * (g.n + 1) + 5 * g.nnz
         shape = {"hub": args.hub, "heavy": args.heavy, "apl": args.apl, "grid": grid}
-        traffic, traffic_src = read_traffic(args.traffic, shape)
+        traffic, traffic_rng, traffic_src = read_traffic(args.traffic, shape)
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": None if traffic is None else round(traffic, 1),
                     "traffic_source": traffic_src,
+                    "traffic_range": None if traffic_rng is None else [round(x, 1) for x in traffic_rng],
                     "achieved_counter_GBs": None if traffic is None else
                     round(traffic / (launch_us * 1e-6) / 1e9, 2),
                     "frac_counter": None if traffic is None else
@@ -365,6 +366,36 @@ def _run_legs(eng, args, legs, out):
         out["serving"] = serving_leg(eng, args)
 
 
+def ghost_discovery(eng, g, local_ids, n_ranks=8):
+    """The partition's ghost topology found again from its global neighbour
+    ids (ss_ghost_discover: ghost ids and per-rank receive lists, host arrays
+    in, size query and fill both timed), checked against the ghost ids the
+    generator wrote and the owner of every id, and timed beside numpy's
+    unique over the same ids (the shape of the reference's own host step)."""
+    col = g.col_idx
+    gid = np.where(col < g.n, local_ids[np.minimum(col, g.n - 1)], g.extra[np.maximum(col - g.n, 0)])
+    owner = (np.arange(g.n * n_ranks, dtype=np.int64) // g.n).astype(np.int32)
+    my_rank = int(local_ids[0] // g.n)
+    eng.ghost_discover(gid[:1024], owner, my_rank, n_ranks)      # first launches outside the timing
+    t0 = time.perf_counter()
+    ids, offsets, index = eng.ghost_discover(gid, owner, my_rank, n_ranks)
+    t_gpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    want = np.unique(gid[owner[gid] != my_rank])
+    t_np = time.perf_counter() - t0
+    ok = bool(np.array_equal(ids, want) and np.array_equal(ids, np.sort(g.extra)))
+    for r in range(n_ranks):
+        pos = index[offsets[r]:offsets[r + 1]]
+        ok = ok and bool(np.all(owner[ids[pos]] == r) and np.all(np.diff(pos) > 0))
+    ok = ok and int(offsets[-1]) == ids.shape[0]
+    log(f"partition: ghost discovery over {gid.shape[0]} entries: {ids.shape[0]} ghosts in {t_gpu * 1e3:.1f} ms "
+        f"(numpy unique {t_np * 1e3:.0f} ms), exact {ok}")
+    return {"entries": int(gid.shape[0]), "ghosts": int(ids.shape[0]), "ms": round(t_gpu * 1e3, 2),
+            "numpy_unique_ms": round(t_np * 1e3, 1), "exact": ok,
+            "note": "ss_ghost_discover from host arrays (int64 ids, int32 owner table of "
+                    f"{owner.shape[0]} agents), size query and fill calls, PCIe included"}
+
+
 def partition_leg(eng, args, deg_out=2.0):
     """One partition's share of a 32M-agent stochastic block model (64 blocks,
     mean degree 12 inside a block and 2 across, 8 parts, part 0): local rows,
@@ -392,6 +423,7 @@ def partition_leg(eng, args, deg_out=2.0):
     t_up = time.perf_counter() - t0
     log(f"partition: {g.n} local agents, {n_ghost} ghosts, {g.nnz} entries, built in {t_gen:.1f}s, "
         f"uploaded in {t_up:.1f}s")
+    disc = ghost_discovery(eng, g, pw.local_ids)
     chunk = 10
     t_run = 0.0
     exact = None
@@ -440,7 +472,7 @@ def partition_leg(eng, args, deg_out=2.0):
             "local_agent_ticks_per_s": rate, "halo_records": int(n_halo),
             "halo_bytes_per_tick": int((n_halo + 3) // 4), "halo_format": "2-bit state codes, four records a byte",
             "oracle_window_ticks": chunk, "oracle_window_exact": exact, "oracle_whole_run_exact": whole,
-            "final_SIR": final,
+            "final_SIR": final, "ghost_discovery": disc,
             "graph_build_s": round(t_gen, 2), "upload_s": round(t_up, 2)}
 
 

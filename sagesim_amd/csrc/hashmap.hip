@@ -47,28 +47,35 @@ __global__ void map_round_claim_kernel(const unsigned long long *__restrict__ ke
 }
 
 // the lowest bidder of a slot takes it (and clears the bid for the next
-// round); every other remaining entry moves to the next slot
-__global__ void map_round_place_kernel(unsigned long long *keys, int32_t *vals, int64_t cap,
-                                       const int64_t *__restrict__ ids, const int32_t *__restrict__ slot_vals,
-                                       int64_t n, int64_t *probe, int32_t *claim, int32_t *remaining) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool left = false;
-    if (g < n) {
+// round); every other remaining entry moves to the next slot.  Grid-stride
+// over the entries with a bounded grid, the count left summed in LDS: one
+// global atomic per block, not one per wavefront on a single address
+// (those serialise at the L2 and cost ~1 ms per round over 6M entries).
+__global__ __launch_bounds__(256) void map_round_place_kernel(unsigned long long *keys, int32_t *vals, int64_t cap,
+                                                              const int64_t *__restrict__ ids,
+                                                              const int32_t *__restrict__ slot_vals, int64_t n,
+                                                              int64_t *probe, int32_t *claim, int32_t *remaining) {
+    __shared__ int32_t block_left;
+    if (threadIdx.x == 0) block_left = 0;
+    __syncthreads();
+    int32_t left = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += stride) {
         const int64_t s = probe[g];
-        if (s >= 0) {
-            if (claim[s] == (int32_t)g) {
-                keys[s] = (unsigned long long)ids[g];
-                vals[s] = slot_vals[g];
-                claim[s] = INT32_MAX;
-                probe[g] = -1;
-            } else {
-                probe[g] = s + 1 == cap ? 0 : s + 1;
-                left = true;
-            }
+        if (s < 0) continue;
+        if (claim[s] == (int32_t)g) {
+            keys[s] = (unsigned long long)ids[g];
+            vals[s] = slot_vals[g];
+            claim[s] = INT32_MAX;
+            probe[g] = -1;
+        } else {
+            probe[g] = s + 1 == cap ? 0 : s + 1;
+            ++left;
         }
     }
-    const unsigned long long b = __ballot(left);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(remaining, (int32_t)__popcll(b));
+    if (left) atomicAdd(&block_left, left);
+    __syncthreads();
+    if (threadIdx.x == 0 && block_left) atomicAdd(remaining, block_left);
 }
 
 // ---- sequential operations on one wavefront ----
@@ -159,7 +166,8 @@ hipError_t ss_launch_map_build_round(unsigned long long *keys, int32_t *vals, in
                                      int32_t *remaining, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(map_round_claim_kernel, dim3(grid_of(n, 256)), dim3(256), 0, s, keys, n, probe, claim);
-    hipLaunchKernelGGL(map_round_place_kernel, dim3(grid_of(n, 256)), dim3(256), 0, s, keys, vals, cap, ids,
+    const unsigned place_grid = grid_of(n, 256) < 2048u ? grid_of(n, 256) : 2048u;
+    hipLaunchKernelGGL(map_round_place_kernel, dim3(place_grid), dim3(256), 0, s, keys, vals, cap, ids,
                        slot_vals, n, probe, claim, remaining);
     return hipGetLastError();
 }
