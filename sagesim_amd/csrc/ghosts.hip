@@ -40,11 +40,10 @@ __global__ void ghost_popc_kernel(const uint32_t *__restrict__ bits, int64_t n_w
 }
 
 // ids of the set bits of word w at positions pos[w] ..; their owner ranks;
-// ranks outside [0, n_ranks) are counted in *bad
+// ranks outside [0, n_ranks) are counted in *bad (an error: the host stops)
 __global__ void ghost_emit_kernel(const uint32_t *__restrict__ bits, int64_t n_words, const int32_t *__restrict__ pos,
                                   const int32_t *__restrict__ owner, int32_t n_ranks, int64_t *__restrict__ ids,
-                                  int32_t *__restrict__ rank, int32_t *__restrict__ index,
-                                  int32_t *__restrict__ rank_count, int32_t *bad) {
+                                  int32_t *__restrict__ rank, int32_t *__restrict__ index, int32_t *bad) {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= n_words) return;
     uint32_t b = bits[w];
@@ -61,10 +60,22 @@ __global__ void ghost_emit_kernel(const uint32_t *__restrict__ bits, int64_t n_w
             rank[p] = 0;
         } else {
             rank[p] = r;
-            atomicAdd(&rank_count[r], 1);
         }
         ++p;
     }
+}
+
+// Receive offsets from the ranks sorted ascending: offsets[q] is the first
+// position whose rank is >= q (offsets[n_ranks] = total).  Each position
+// where the rank steps up writes the offsets of the ranks it steps over, so
+// no counter is shared (per-rank atomic counts serialise on a few addresses).
+__global__ void ghost_offsets_kernel(const int32_t *__restrict__ sorted_rank, int64_t total, int32_t n_ranks,
+                                     int32_t *__restrict__ offsets) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > total) return;
+    const int32_t lo = p == 0 ? -1 : sorted_rank[p - 1];
+    const int32_t hi = p == total ? n_ranks : sorted_rank[p];
+    for (int32_t q = lo + 1; q <= hi; ++q) offsets[q] = (int32_t)p;
 }
 
 int bits_for(int32_t n) {
@@ -123,13 +134,13 @@ extern "C" int ss_ghost_discover(ss_ctx *ctx, const int64_t *nbr_ids, int64_t nn
     if (e == hipSuccess) e = s.get(&d_bits, (size_t)n_words);
     if (e == hipSuccess) e = s.get(&d_cnt, (size_t)n_words + 1);
     if (e == hipSuccess) e = s.get(&d_pos, (size_t)n_words + 1);
-    if (e == hipSuccess) e = s.get(&d_misc, 1 + (size_t)n_ranks);
+    if (e == hipSuccess) e = s.get(&d_misc, 1);
     if (e == hipSuccess && nnz) e = hipMemcpyAsync(d_nbr, nbr_ids, sizeof(int64_t) * nnz, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && n_owner)
         e = hipMemcpyAsync(d_owner, owner, sizeof(int32_t) * n_owner, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemsetAsync(d_bits, 0, sizeof(uint32_t) * (size_t)std::max<int64_t>(n_words, 1), st);
     if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, sizeof(int32_t) * (size_t)(n_words + 1), st);
-    if (e == hipSuccess) e = hipMemsetAsync(d_misc, 0, sizeof(int32_t) * (size_t)(1 + n_ranks), st);
+    if (e == hipSuccess) e = hipMemsetAsync(d_misc, 0, sizeof(int32_t), st);
     if (e == hipSuccess && nnz) {
         const int grid = (int)std::min<int64_t>((nnz + 255) / 256, (int64_t)ctx->num_cus * 16);
         hipLaunchKernelGGL(ghost_mark_kernel, dim3(grid), dim3(256), 0, st, d_nbr, nnz, d_owner, n_owner, my_rank,
@@ -159,15 +170,16 @@ extern "C" int ss_ghost_discover(ss_ctx *ctx, const int64_t *nbr_ids, int64_t nn
     if (!ghost_ids) return SS_OK;
     if (cap < total) return fail(ctx, SS_ERR_INVALID, "ss_ghost_discover: cap is below the number of ghosts");
     int64_t *d_ids = nullptr;
-    int32_t *d_rank = nullptr, *d_rank2 = nullptr, *d_idx = nullptr, *d_idx2 = nullptr;
+    int32_t *d_rank = nullptr, *d_rank2 = nullptr, *d_idx = nullptr, *d_idx2 = nullptr, *d_off = nullptr;
     e = s.get(&d_ids, (size_t)total);
+    if (e == hipSuccess) e = s.get(&d_off, (size_t)n_ranks + 1);
     if (e == hipSuccess) e = s.get(&d_rank, (size_t)total);
     if (e == hipSuccess) e = s.get(&d_rank2, (size_t)total);
     if (e == hipSuccess) e = s.get(&d_idx, (size_t)total);
     if (e == hipSuccess) e = s.get(&d_idx2, (size_t)total);
     if (e == hipSuccess && n_words) {
         hipLaunchKernelGGL(ghost_emit_kernel, dim3((unsigned)((n_words + 255) / 256)), dim3(256), 0, st, d_bits, n_words,
-                           d_pos, d_owner, n_ranks, d_ids, d_rank, d_idx, d_misc + 1, d_misc);
+                           d_pos, d_owner, n_ranks, d_ids, d_rank, d_idx, d_misc);
         e = hipGetLastError();
     }
     // ghost positions grouped by owner rank, id order kept (the sort is stable)
@@ -181,9 +193,16 @@ extern "C" int ss_ghost_discover(ss_ctx *ctx, const int64_t *nbr_ids, int64_t nn
             e = hipcub::DeviceRadixSort::SortPairs(d_sort, sb, d_rank, d_rank2, d_idx, d_idx2, (int)total, 0,
                                                    bits_for(n_ranks), st);
     }
-    std::vector<int32_t> misc((size_t)n_ranks + 1);
+    if (e == hipSuccess) {
+        // ranks sorted into d_rank2 (total == 0: every offset is 0)
+        hipLaunchKernelGGL(ghost_offsets_kernel, dim3((unsigned)((total + 1 + 255) / 256)), dim3(256), 0, st,
+                           total ? d_rank2 : d_rank, (int64_t)total, n_ranks, d_off);
+        e = hipGetLastError();
+    }
+    int32_t bad = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_misc, sizeof(int32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(misc.data(), d_misc, sizeof(int32_t) * misc.size(), hipMemcpyDeviceToHost, st);
+        e = hipMemcpyAsync(recv_offsets, d_off, sizeof(int32_t) * ((size_t)n_ranks + 1), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && total) e = hipMemcpyAsync(ghost_ids, d_ids, sizeof(int64_t) * total, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && total)
         e = hipMemcpyAsync(recv_index, d_idx2, sizeof(int32_t) * total, hipMemcpyDeviceToHost, st);
@@ -193,8 +212,6 @@ extern "C" int ss_ghost_discover(ss_ctx *ctx, const int64_t *nbr_ids, int64_t nn
         std::snprintf(b, sizeof b, "ss_ghost_discover: %s", hipGetErrorString(e));
         return fail(ctx, e == hipErrorOutOfMemory ? SS_ERR_OOM : SS_ERR_HIP, b);
     }
-    if (misc[0]) return fail(ctx, SS_ERR_RANGE, "ss_ghost_discover: an owner rank is >= n_ranks");
-    recv_offsets[0] = 0;
-    for (int32_t r = 0; r < n_ranks; ++r) recv_offsets[r + 1] = recv_offsets[r] + misc[(size_t)r + 1];
+    if (bad) return fail(ctx, SS_ERR_RANGE, "ss_ghost_discover: an owner rank is >= n_ranks");
     return SS_OK;
 }

@@ -69,6 +69,23 @@ def test_discover_matches_brute_force(eng):
     assert discover_ghost_topology([], d, my, engine=eng).shape == (0,)
 
 
+def test_discover_offsets_with_empty_and_trailing_ranks(eng):
+    """recv_offsets straight from ss_ghost_discover: ranks with no ghosts in
+    the middle and at the end, and no ghosts at all."""
+    owner = np.array([0, 0, 2, 2, 2, 5, 1, 1], np.int32)
+    ids, off, idx = eng.ghost_discover(np.array([7, 2, 2, 4, 0, 3, 6], np.int64), owner, 1, 8)
+    assert ids.tolist() == [0, 2, 3, 4]              # 6 and 7 are rank 1's own
+    assert off.tolist() == [0, 1, 1, 4, 4, 4, 4, 4, 4] and idx.tolist() == [0, 1, 2, 3]
+    ids, off, idx = eng.ghost_discover(np.array([7, 2, 5, 0, 3, 6], np.int64), owner, 1, 8)
+    assert ids.tolist() == [0, 2, 3, 5]
+    assert off.tolist() == [0, 1, 1, 3, 3, 3, 4, 4, 4]
+    assert idx.tolist() == [0, 1, 2, 3]
+    ids, off, idx = eng.ghost_discover(np.array([6, 7, 6], np.int64), owner, 1, 8)
+    assert ids.shape == (0,) and off.tolist() == [0] * 9 and idx.shape == (0,)
+    ids, off, idx = eng.ghost_discover(np.zeros(0, np.int64), owner, 1, 3)
+    assert ids.shape == (0,) and off.tolist() == [0] * 4
+
+
 def test_discover_rejects_ranks_beyond_count(eng):
     from sagesim_amd._lib import NativeError
     owner = np.array([0, 1, 9], np.int32)
