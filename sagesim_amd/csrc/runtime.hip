@@ -112,6 +112,12 @@ int apply_model(ss_ctx *ctx, ss_replica &r, const ss_model *m) {
     r.dev.inf_code = (int8_t)m->inf_code;
     r.dev.s_target = (int8_t)m->s_target;
     r.dev.n_states = (int8_t)m->n_states;
+    // an agent that is not susceptible at tick t is not at t + 1 either when
+    // infection leaves s_code and no transition enters it: the pushed counts
+    // of such agents would never be read
+    bool back = m->s_target == m->s_code;
+    for (int s = 0; s < 4; ++s) back |= r.dev.trans_T[s] != 0u && r.dev.trans_next[s] == r.dev.s_code;
+    r.dev.push_sus_only = back ? 0u : 1u;
     return SS_OK;
 }
 

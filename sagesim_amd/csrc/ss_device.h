@@ -24,6 +24,7 @@ struct ss_rep_dev {
     const uint32_t *inf_T;      // inf_T[k] = ceil(p_k * 2^23), k = #infectious neighbours
     int32_t inf_T_last;         // last index of inf_T (k is clamped to it)
     uint32_t push_ok;           // CSR is symmetric and there are no ghosts: candidates may be pushed
+    uint32_t push_sus_only;     // no state returns to s_code: counts go only to agents susceptible now
     uint32_t trans_T[4];        // spontaneous transition thresholds per state (0 = none)
     int8_t trans_next[4];       // ...and target states
     int8_t s_code, inf_code, s_target, n_states;

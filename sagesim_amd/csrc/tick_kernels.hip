@@ -72,6 +72,8 @@ struct Cold {
     int64_t n_hub, h64;          // agent -> tile mapping of the replica
     int32_t hub0, heavy0, light0;          // tag slots (see ss_rep_dev)
     uint8_t ptag;                // tile tag of the next tick
+    const int8_t *pcur;          // states of this tick when counts go only to susceptible agents, else null
+    int8_t s_code;
 };
 
 typedef SS_L Cold LCold;        // the record as it sits in LDS
@@ -81,12 +83,14 @@ typedef SS_L Cold LCold;        // the record as it sits in LDS
 struct Pusher {
     uint16_t *pcnt;
     SS_G uint8_t *tnext;
+    const SS_G int8_t *cur;      // non-null: push only to agents susceptible now
     int64_t n_hub, h64;
     int32_t hub0, heavy0, light0;
     uint8_t ptag;
+    int8_t s_code;
     __device__ __forceinline__ explicit Pusher(const LCold *C)
-        : pcnt(C->pcnt), tnext((SS_G uint8_t *)C->tnext), n_hub(C->n_hub), h64(C->h64),
-          hub0(C->hub0), heavy0(C->heavy0), light0(C->light0), ptag(C->ptag) {}
+        : pcnt(C->pcnt), tnext((SS_G uint8_t *)C->tnext), cur((const SS_G int8_t *)C->pcur), n_hub(C->n_hub),
+          h64(C->h64), hub0(C->hub0), heavy0(C->heavy0), light0(C->light0), ptag(C->ptag), s_code(C->s_code) {}
     // tag slot of agent v (see ss_rep_dev)
     __device__ __forceinline__ int64_t slot_of(int64_t v) const {
         if (v < n_hub) return hub0 + v;
@@ -98,16 +102,43 @@ struct Pusher {
         ss_count_add(pcnt, c);
         tnext[slot_of(c)] = ptag;
     }
-    // entries [b, e) of col, by `width` threads from `me` on, four loads in
-    // flight per thread
+    // entries [b, e) of col, by `width` threads from `me` on, four per thread
+    // and round.  The entries of the next round are loaded before this round's
+    // atomics and tag stores are issued: a wait for those loads then does not
+    // wait for the atomics (memory operations count down in issue order).
+    // With `cur`, the states of the round's targets are read first and only
+    // susceptible targets are pushed (agents pushed to most often, hubs, are
+    // infected early and would otherwise take every one of their rows'
+    // atomics on one word).
     __device__ __forceinline__ void range(const SS_G int32_t *col, int64_t b, int64_t e, int me, int width) const {
-        for (int64_t j = b + me; j < e; j += 4 * (int64_t)width) {
-            int32_t c[4];
+        int64_t j = b + me;
+        if (j >= e) return;
+        int32_t c[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) c[q] = j + q * width < e ? col[j + q * width] : -1;
+        for (int q = 0; q < 4; ++q) c[q] = j + q * width < e ? col[j + q * width] : -1;
+        for (;;) {
+            const int64_t jn = j + 4 * (int64_t)width;
+            int32_t cn[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cn[q] = jn + q * width < e ? col[jn + q * width] : -1;
+            bool go[4];
+            if (cur) {
+                int8_t st[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) st[q] = c[q] >= 0 ? cur[c[q]] : (int8_t)0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) go[q] = c[q] >= 0 && st[q] == s_code;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) go[q] = c[q] >= 0;
+            }
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if (c[q] >= 0) one(c[q]);
+                if (go[q]) one(c[q]);
+            if (jn >= e) break;
+            j = jn;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) c[q] = cn[q];
         }
     }
 };
@@ -1105,6 +1136,8 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
                     C->heavy0 = R->heavy0;
                     C->light0 = R->light0;
                     C->ptag = ss_front_tag(t1);
+                    C->pcur = R->push_sus_only ? (const int8_t *)((t & 1) != 0 ? R->state[1] : R->state[0]) : nullptr;
+                    C->s_code = R->s_code;
                 }
                 if (threadIdx.x < 4) {
                     C->tT[threadIdx.x] = R->trans_T[threadIdx.x];
