@@ -588,13 +588,16 @@ def serving_leg(eng, args):
         log(f"serving: t={now:.0f}s arrived {arrived}, done {st.completed}, in flight {inflight}, "
             f"queued {ready}, building {building}")
 
+    # the requests re-run alone afterwards are chosen up front; only their
+    # states are kept (a 10-minute window finishes ~30k requests)
+    rng = np.random.default_rng(1)
+    keep = set(rng.choice(len(arrivals), size=min(4, len(arrivals)), replace=False).tolist()) if arrivals else set()
     srv = Server(eng, make, budget_bytes=200 << 30, chunk_ticks=10, builders=max(1, host_threads() - 1),
-                 progress=progress, progress_every_s=30.0)
+                 progress=progress, progress_every_s=30.0, keep_results=keep)
     stats, done = srv.run(arrivals)
     srv.close()
     summ = stats.summary()
-    rng = np.random.default_rng(1)
-    sample = [done[i] for i in rng.choice(len(done), size=min(4, len(done)), replace=False)] if done else []
+    sample = [r for r in done if r.rid in keep and r.outcome == "done"]
     prepared = []
     for r in sample:
         sc = r.scenario
@@ -602,13 +605,15 @@ def serving_leg(eng, args):
         prepared.append(Prepared(sc, g, sc.initial_state()))
     alone, _ = run_standalone(eng, prepared)
     summ["sample_bitexact_alone_vs_mix"] = all(np.array_equal(a, r.result) for a, r in zip(alone, sample))
-    summ["agent_ticks"] = int(sum(r.scenario.n * r.scenario.ticks for r in done if r.result is not None))
+    summ["sample_size"] = len(sample)
+    summ["agent_ticks"] = int(sum(r.scenario.n * r.scenario.ticks for r in done if r.outcome == "done"))
     summ["rate_per_s"] = args.serve_rate
     summ["arrival_window_s"] = args.serve_seconds
     if args.serve_capacity > 0:
         # capacity: every request of a sample there at once (saturating arrivals)
         burst = [(0.0, sc) for _, sc in serving_requests(rate=50.0, duration=args.serve_capacity / 50.0, seed=4242)]
-        srv = Server(eng, make, budget_bytes=200 << 30, chunk_ticks=10, builders=max(1, host_threads() - 1))
+        srv = Server(eng, make, budget_bytes=200 << 30, chunk_ticks=10, builders=max(1, host_threads() - 1),
+                     keep_results=False)
         cst, cdone = srv.run(burst, realtime=False)
         srv.close()
         cs = cst.summary()

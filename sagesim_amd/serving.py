@@ -110,6 +110,9 @@ class Server:
     def __init__(self, engine, make_replica, budget_bytes=200 << 30, chunk_ticks=10, builders=8,
                  keep_results=True, clock=time.perf_counter, builder_threads=1, deadline_s=None,
                  progress=None, progress_every_s=30.0):
+        # keep_results: True (every finished request keeps its states), False
+        # (none), or a collection of request ids whose states are kept (long
+        # windows: tens of thousands of results would fill host memory)
         self.builder_threads = builder_threads
         self.engine = engine
         self.make_replica = make_replica
@@ -236,8 +239,8 @@ class Server:
                 for r in inflight:
                     why = self._leaving(r, now)
                     if why is None and r.replica.tick() >= r.scenario.ticks:
-                        res = r.replica.states()
-                        r.result = res if self.keep else None
+                        keep = self.keep if isinstance(self.keep, bool) else r.rid in self.keep
+                        r.result = r.replica.states() if keep else None
                         r.replica.remove()
                         r.replica = None
                         r.graph = r.state = r.data = None

@@ -176,3 +176,16 @@ def test_deadline_expires_slow_requests():
     assert by[7].outcome == "done"                   # its own deadline is far off
     assert all(by[i].outcome == "done" for i in range(6))
     assert stats.expired == 1 and stats.completed == 7 and eng.live == []
+
+
+def test_results_kept_only_for_chosen_requests():
+    arrivals = _small_requests(8)
+    eng = FakeEngine()
+    srv = Server(eng, lambda req: FakeReplica(eng, req), budget_bytes=1 << 40, chunk_ticks=10, builders=2,
+                 keep_results={1, 5})
+    stats, done = srv.run(arrivals)
+    srv.close()
+    assert stats.completed == len(arrivals)
+    assert sorted(r.rid for r in done if r.result is not None) == [1, 5]
+    assert all((r.result == r.rid % 3).all() for r in done if r.result is not None)
+    assert all(r.outcome == "done" for r in done)
