@@ -53,12 +53,20 @@ def cpu_model():
     return "unknown"
 
 
+def kernel_source_sha():
+    """sha256 of the tick kernel's source: a traffic record is only used for the kernel it measured."""
+    import hashlib
+    with open(os.path.join(ROOT, "sagesim_amd", "csrc", "tick_kernels.hip"), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
 def read_traffic(path, shape):
     """HBM bytes per tick-kernel launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
     from a committed profile summary holding one record per measured launch shape: a line
-    'shape: hub=.. heavy=.. apl=.. grid=..' followed by 'traffic_bytes_per_launch: <float>' and
-    'traffic_bytes_range: <lo> <hi>'.  Returns (bytes, (lo, hi), source) for the record of this
-    run's shape, or (None, None, reason)."""
+    'shape: hub=.. heavy=.. apl=.. grid=..' followed by 'kernel_source_sha256: <hex>',
+    'traffic_bytes_per_launch: <float>' and 'traffic_bytes_range: <lo> <hi>'.  Returns
+    (bytes, (lo, hi), source) for the record of this run's shape and kernel source, or
+    (None, None, reason)."""
     want = {k: str(v) for k, v in shape.items()}
     recs, cur = [], None
     try:
@@ -67,15 +75,24 @@ def read_traffic(path, shape):
                 if line.startswith("shape:"):
                     cur = {"shape": dict(kv.partition("=")[::2] for kv in line.split(":", 1)[1].split())}
                     recs.append(cur)
+                elif cur is not None and line.startswith("kernel_source_sha256:"):
+                    cur["sha"] = line.split(":", 1)[1].strip()
                 elif cur is not None and line.startswith("traffic_bytes_per_launch:"):
                     cur["bytes"] = float(line.split(":", 1)[1])
                 elif cur is not None and line.startswith("traffic_bytes_range:"):
                     cur["range"] = tuple(float(x) for x in line.split(":", 1)[1].split())
     except OSError:
         return None, None, f"no traffic profile at {os.path.relpath(path, ROOT)}"
+    sha = kernel_source_sha()
+    stale = False
     for r in recs:
         if {k: r["shape"].get(k) for k in want} == want and "bytes" in r:
-            return r["bytes"], r.get("range"), "rocprofv3 FETCH_SIZE + WRITE_SIZE of this launch shape, calibrated"
+            if r.get("sha") == sha:
+                return r["bytes"], r.get("range"), "rocprofv3 FETCH_SIZE + WRITE_SIZE of this launch shape, calibrated"
+            stale = True
+    if stale:
+        return None, None, (f"the traffic record for launch shape {want} in {os.path.relpath(path, ROOT)} "
+                            "measured another version of tick_kernels.hip")
     return None, None, f"no traffic record for launch shape {want} in {os.path.relpath(path, ROOT)}"
 
 

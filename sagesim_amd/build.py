@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 ARCH = os.environ.get("SAGESIM_OFFLOAD_ARCH", "gfx950")
 
-STEP_SOURCES = ["tick_kernels.hip", "hashmap.hip", "runtime.hip", "noalloc.hip", "ghosts.hip", "rtc.hip"]
+STEP_SOURCES = ["tick_kernels.hip", "aux_kernels.hip", "hashmap.hip", "runtime.hip", "noalloc.hip", "ghosts.hip", "rtc.hip"]
 # Packed-FP32 instructions are kept out of every device code object.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -1108,7 +1108,8 @@ int ss_halo_bind(ss_ctx *ctx, int32_t rep, const int64_t *ids, int64_t n, int64_
     for (int32_t s : slots)
         if (s >= 0) lo = std::min<int64_t>(lo, s);
     r.halo_base = lo / 64 * 64;
-    std::vector<int32_t> rec((size_t)(r.n_total - r.halo_base), -1);
+    // padded to whole groups of four slots (the refresh reads four records at once)
+    std::vector<int32_t> rec((size_t)((r.n_total - r.halo_base + 3) / 4 * 4), -1);
     for (int64_t h = 0; h < n; ++h)
         if (slots[(size_t)h] >= 0) rec[(size_t)(slots[(size_t)h] - r.halo_base)] = (int32_t)h;
     SS_HIP(ctx, dalloc(ctx, &r, &r.halo_rec, rec.size()));
