@@ -116,6 +116,15 @@ __device__ inline double rand_uniform_philox4x32(long long tick, const AgentInde
 // and the loop's loads can overlap.  Writes to any other slot, and every
 // write to a no_double_buffer property or a row, go to memory at once.
 // Reads always come from the read buffer, as in the reference.
+// Device memory is read and written through global-address-space pointers,
+// so loads and stores are global_* (not flat_*, which also count on the LDS
+// counter and take the flat aperture check); the proxies keep plain pointers.
+#define SS_RT_G __attribute__((address_space(1)))
+template <class T>
+__device__ inline T ss_ld(const T *p) { return *(const SS_RT_G T *)p; }
+template <class T>
+__device__ inline void ss_st(T *p, T v) { *(SS_RT_G T *)p = v; }
+
 template <class T>
 struct OwnSlot {
     T v;
@@ -127,21 +136,21 @@ struct PropRef {
     const T *r;
     T *w;
     OwnSlot<T> *own;      // non-null: this is the running agent's own slot
-    __device__ operator T() const { return *r; }
+    __device__ operator T() const { return ss_ld(r); }
     __device__ void store(T v) const {
         if (own) {
             own->v = v;
             own->dirty = true;
         } else {
-            *w = v;
+            ss_st(w, v);
         }
     }
-    __device__ const PropRef &operator=(const PropRef &o) const { store((T)(*o.r)); return *this; }
+    __device__ const PropRef &operator=(const PropRef &o) const { store((T)ss_ld(o.r)); return *this; }
     template <class V> __device__ const PropRef &operator=(V v) const { store((T)v); return *this; }
-    template <class V> __device__ const PropRef &operator+=(V v) const { store((T)(*r + v)); return *this; }
-    template <class V> __device__ const PropRef &operator-=(V v) const { store((T)(*r - v)); return *this; }
-    template <class V> __device__ const PropRef &operator*=(V v) const { store((T)(*r * v)); return *this; }
-    template <class V> __device__ const PropRef &operator/=(V v) const { store((T)(*r / v)); return *this; }
+    template <class V> __device__ const PropRef &operator+=(V v) const { store((T)(ss_ld(r) + v)); return *this; }
+    template <class V> __device__ const PropRef &operator-=(V v) const { store((T)(ss_ld(r) - v)); return *this; }
+    template <class V> __device__ const PropRef &operator*=(V v) const { store((T)(ss_ld(r) * v)); return *this; }
+    template <class V> __device__ const PropRef &operator/=(V v) const { store((T)(ss_ld(r) / v)); return *this; }
 };
 
 template <class T>
@@ -159,7 +168,7 @@ struct Prop {
 // store a held own-slot write (called by the wrapper after the step returns)
 template <class T>
 __device__ inline void ss_flush(const Prop<T> &p, const OwnSlot<T> &o) {
-    if (o.dirty) p.w[p.self] = o.v;
+    if (o.dirty) ss_st(p.w + p.self, o.v);
 }
 
 template <class T>
@@ -231,14 +240,14 @@ __device__ inline long long len(const BlaRows2<T> &x) { return x.rows; }
 // buffer index -> row of a breed-local array (-1 for agents of other breeds)
 struct IdxMap {
     const int *p;
-    __device__ int operator[](long long i) const { return p[i]; }
+    __device__ int operator[](long long i) const { return ss_ld(p + i); }
 };
 
 // ---- network (CSR) ----
 struct NeighborRow {
     const int *p;
     int n;
-    __device__ int operator[](long long j) const { return p[j]; }
+    __device__ int operator[](long long j) const { return ss_ld(p + j); }
 };
 __device__ inline int len(const NeighborRow &row) { return row.n; }
 
@@ -246,14 +255,14 @@ struct Locations {
     const long long *row_ptr;
     const int *col_idx;
     __device__ NeighborRow operator[](long long i) const {
-        const long long b = row_ptr[i];
-        return NeighborRow{col_idx + b, (int)(row_ptr[i + 1] - b)};
+        const long long b = ss_ld(row_ptr + i);
+        return NeighborRow{col_idx + b, (int)(ss_ld(row_ptr + i + 1) - b)};
     }
 };
 
 struct FloatArray {
     const float *p;
-    __device__ float operator[](long long i) const { return p[i]; }
+    __device__ float operator[](long long i) const { return ss_ld(p + i); }
 };
 
 struct GlobalArray {
