@@ -58,6 +58,12 @@ def build_csr_values_only(ragged_list, offsets):
     return build_csr_from_ragged(ragged_list)[1][: int(offsets[-1])]
 
 
+
+# Rows of the hipRTC path that start first (longest first): degree at least
+# _LONG_ROW_FACTOR times the priority group's mean and at least _LONG_ROW_MIN.
+_LONG_ROW_FACTOR = 8.0
+_LONG_ROW_MIN = 64.0
+
 class _Program:
     """Owner of one ss_prog (general step functions)."""
 
@@ -657,7 +663,9 @@ class Model:
             raise NativeError(rc, L.ss_prog_last_error(h).decode())
         self._push_seed()
         # schedule: priority group p covers the breed-sorted range of its breeds
-        # (model.py:1051-1062); within it the longest rows start first
+        # (model.py:1051-1062); within it the long rows (see _LONG_ROW_FACTOR)
+        # start first, longest first, and the rest follow in buffer order, so
+        # that most threads read their agent's fields from consecutive slots
         deg = np.diff(np.asarray(rp, np.int64))
         kern, start, count, segs = [], [], [], []
         for p, grp in enumerate(groups):
@@ -671,7 +679,12 @@ class Model:
             kern.append(p)
             start.append(lo)
             count.append(hi - lo)
-            segs.append(lo + np.argsort(-deg[lo:hi], kind="stable"))
+            d = deg[lo:hi]
+            cut = max(_LONG_ROW_MIN, _LONG_ROW_FACTOR * float(d.mean())) if d.size else 0.0
+            long_rows = np.nonzero(d >= cut)[0]
+            long_rows = long_rows[np.argsort(-d[long_rows], kind="stable")]
+            rest = np.nonzero(d < cut)[0]
+            segs.append(lo + np.concatenate([long_rows, rest]))
         k = np.asarray(kern, np.int32)
         s = np.asarray(start, np.int64)
         c = np.asarray(count, np.int64)
