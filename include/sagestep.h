@@ -184,7 +184,7 @@ int ss_map_lookup(ss_ctx *ctx, int32_t rep, const int64_t *ids, int32_t *out, in
  * ghost_ids of the ghosts r owns, ascending (the per-rank np.unique of
  * gpu_kernels.py:610-617).  recv_offsets has n_ranks + 1 entries.  With
  * ghost_ids NULL only *n_ghost is set; an owner rank >= n_ranks is
- * SS_ERR_RANGE.  The send side is ss_map_lookup of the ids a peer requests in
+ * SS_ERR_RANGE; n_ranks is at most 2^20 and n_owner at most 2^31 - 1.  The send side is ss_map_lookup of the ids a peer requests in
  * a map built over the local (and ghost) agents (model.py:1112-1115). */
 int ss_ghost_discover(ss_ctx *ctx, const int64_t *nbr_ids, int64_t nnz, const int32_t *owner, int64_t n_owner,
                       int32_t my_rank, int32_t n_ranks, int64_t *ghost_ids, int32_t *recv_offsets,

@@ -117,8 +117,9 @@ extern "C" int ss_ghost_discover(ss_ctx *ctx, const int64_t *nbr_ids, int64_t nn
     if (!ctx) return SS_ERR_INVALID;
     if (!n_ghost || nnz < 0 || n_owner < 0 || (nnz && !nbr_ids) || (n_owner && !owner))
         return fail(ctx, SS_ERR_INVALID, "ss_ghost_discover: NULL argument or negative size");
-    if (n_ranks < 1 || n_ranks > (1 << 20) || n_owner > (1ll << 35))
-        return fail(ctx, SS_ERR_INVALID, "ss_ghost_discover: n_ranks in [1, 2^20], at most 2^35 ids");
+    // ghost positions and counts are int32: at most INT32_MAX possible ghost ids
+    if (n_ranks < 1 || n_ranks > (1 << 20) || n_owner > (int64_t)INT32_MAX)
+        return fail(ctx, SS_ERR_INVALID, "ss_ghost_discover: n_ranks in [1, 2^20], n_owner at most 2^31 - 1");
     if (ghost_ids && (!recv_offsets || !recv_index))
         return fail(ctx, SS_ERR_INVALID, "ss_ghost_discover: recv_offsets and recv_index are needed with ghost_ids");
     (void)hipSetDevice(ctx->device);

@@ -263,3 +263,46 @@ def test_row_of_65535_or_more_stays_pull(engines):
     e.simulate(20)
     assert first_diff(rep.states(), want) is None
     rep.remove()
+
+
+def test_sirs_counts_reach_every_neighbour(engines):
+    """SIRS (R -> S with probability 0.05): a recovered agent becomes
+    susceptible again, so pushed counts must reach neighbours that are not
+    susceptible when they are pushed.  The three rules agree with the oracle
+    every few ticks over a run long enough for agents to return to S."""
+    from sagesim_amd._lib import SSModel
+    g = graphs.degree_sort(graphs.barabasi_albert(60_000, 4, 5))[0]
+    st = workloads.seed_infected(g.n, 300, 9)
+    beta, gamma, xi, seed = 0.06, 0.15, 0.05, 77
+    m = SSModel()
+    m.n_states, m.s_code, m.inf_code, m.s_target = 3, 0, 1, 1
+    m.beta = beta
+    m.trans_p[:] = [0.0, gamma, xi, 0.0]
+    m.trans_next[:] = [0, 2, 0, 0]
+    om = O.sir_model(beta, gamma, seed)
+    om.trans_p[:] = [0.0, gamma, xi, 0.0]
+    om.trans_next[:] = [0, 2, 0, 0]
+    ticks = 120
+    say("SIRS push", g.n, g.nnz)
+    _, trace = O.sir_run(om, g.row_ptr, g.col_idx, np.arange(g.n), st, ticks, trace=True)
+    reps = [Replica(e, g, st, m, seed) for e in engines]
+    assert all(e.push_ok(r.rep) for e, r in zip(engines, reps))
+    for e in engines:
+        e.set_counting(True)
+        e.read_counters()
+    for t in range(ticks):
+        for e in engines:
+            e.simulate(1)
+        if t % 6 == 5:
+            for r in reps:
+                got = r.states()
+                assert first_diff(got, trace[t]) is None, (t, first_diff(got, trace[t]))
+    push_ticks = [int(e.read_counters()[5]) for e in engines]
+    for e in engines:
+        e.set_counting(False)
+    say("push ticks per rule", push_ticks)
+    assert push_ticks[2] == ticks
+    # agents did come back: some susceptible agents had been recovered earlier
+    assert (trace[-1] == 0).sum() > 0 and (trace[ticks // 2] == 2).sum() > 0
+    for r in reps:
+        r.remove()
