@@ -405,3 +405,32 @@ def test_halo_2bit_pack_unpack_all_codes(eng):
     assert first_diff(got[1][0], got[0][0]) is None
     assert np.array_equal(got[1][2], pack_2bit(got[1][1]))
     assert np.array_equal(got[0][2], got[1][2])
+
+
+def test_plain_ticks_after_halo_ticks(eng):
+    """Ticks fed by a halo (whose ghost-only tiles the tick skips), then plain
+    ticks on the same replica (every tile runs, ghosts keep the last halo's
+    states): states and history equal the oracle's two-phase run."""
+    pw = workloads.sbm_partition(n_blocks=8, block_size=20000, deg_in=10, deg_out=3, n_parts=2, part=1,
+                                 ticks=24, seed=41)
+    g = pw.graph
+    t1 = 14
+    say("halo then plain", g.n, g.n_cols, g.nnz)
+    vals = pw.halo_vals(0, t1)
+    hist = pw.history.copy()
+    om = O.sir_model(pw.beta, pw.gamma, pw.seed)
+    mid, _ = O.sir_run(om, g.row_ptr, g.col_idx, pw.local_ids, pw.state, t1, n_local=g.n, history=hist,
+                       ghost_ids=pw.ghost_ids, halo_ids=pw.halo_ids, halo_vals=vals)
+    want, _ = O.sir_run(om, g.row_ptr, g.col_idx, pw.local_ids, mid, pw.ticks - t1, tick0=t1, n_local=g.n,
+                        history=hist)
+    rep = Replica(eng, g, pw.state, sir_model(pw.beta, pw.gamma), pw.seed, rng_key_id=pw.local_ids,
+                  history=pw.history)
+    eng.map_build(rep.rep, pw.ghost_ids, np.arange(g.n, g.n_cols, dtype=np.int32), 0.5)
+    assert eng.halo_bind(rep.rep, pw.halo_ids) == 0
+    eng.simulate_halo(rep.rep, t1, vals)
+    assert first_diff(rep.states(), mid) is None
+    eng.simulate(pw.ticks - t1)
+    got = rep.states()
+    assert first_diff(got, want) is None, first_diff(got, want)
+    assert np.array_equal(rep.history(), hist)
+    rep.remove()
