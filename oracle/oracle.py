@@ -39,6 +39,8 @@ def lib():
         L.or_infection_probs.restype = None
         L.or_sir_run.argtypes = [P, i64, i64, P, P, P, P, P, i64, i64, P, P, P, i64, P, ctypes.c_int]
         L.or_sir_run.restype = ctypes.c_int
+        L.or_sir_run_fast.argtypes = L.or_sir_run.argtypes
+        L.or_sir_run_fast.restype = ctypes.c_int
         L.or_ref_sir_run.argtypes = [i64, P, P, P, P, i32, i64, i64, dbl, dbl]
         L.or_ref_sir_run.restype = None
         _lib = L
@@ -80,8 +82,9 @@ def _p(a):
 
 def sir_run(model, row_ptr, col_idx, rng_id, state, ticks, tick0=0, n_local=None,
             history=None, ghost_ids=None, halo_ids=None, halo_vals=None,
-            trace=False, nthreads=0):
-    """Run the oracle; returns (final_state, trace or None).  `state` is not modified."""
+            trace=False, nthreads=0, fast=False):
+    """Run the oracle; returns (final_state, trace or None).  `state` is not modified.
+    fast=True runs or_sir_run_fast, the tuned restatement with the same results."""
     row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int64)
     col_idx = np.ascontiguousarray(col_idx, dtype=np.int32)
     rng_id = np.ascontiguousarray(rng_id, dtype=np.int64)
@@ -99,9 +102,10 @@ def sir_run(model, row_ptr, col_idx, rng_id, state, ticks, tick0=0, n_local=None
     tr = np.empty((ticks, n_total), dtype=np.int8) if trace else None
     if history is not None:
         assert history.dtype == np.int32 and history.flags.c_contiguous
-    rc = lib().or_sir_run(ctypes.byref(model), n_local, n_total, _p(row_ptr), _p(col_idx),
-                          _p(rng_id), _p(st), _p(history), tick0, ticks,
-                          _p(ghost_ids), _p(halo_ids), _p(halo_vals), n_halo, _p(tr), nthreads)
+    fn = lib().or_sir_run_fast if fast else lib().or_sir_run
+    rc = fn(ctypes.byref(model), n_local, n_total, _p(row_ptr), _p(col_idx),
+            _p(rng_id), _p(st), _p(history), tick0, ticks,
+            _p(ghost_ids), _p(halo_ids), _p(halo_vals), n_halo, _p(tr), nthreads)
     if rc != 0:
         raise ValueError("oracle: halo id names no ghost")
     return st, tr

@@ -5,8 +5,10 @@
  *
  * It is a plain restatement of the reference semantics, written for being
  * obviously right rather than fast (one pass over every agent, every
- * neighbour, every tick; OpenMP over agents only).  Citations are relative to
- * the reference repository root.
+ * neighbour, every tick; OpenMP over agents only).  or_sir_run_fast is the
+ * exception: the same results with the work a tuned CPU code would skip left
+ * out, timed as the CPU baseline.  Citations are relative to the reference
+ * repository root.
  *
  * Pinning (what checks each function against something other than itself):
  *   or_philox2x32_ref   - tests/test_oracle_rng.py: pure-Python big-int
@@ -21,6 +23,9 @@
  *                          p=1 BFS front), invariants (monotone S->I->R,
  *                          counts conserved) and brute force on tiny graphs
  *                          written independently in tests/test_oracle_sir.py.
+ *   or_sir_run_fast     - or_sir_run on the same inputs, bit for bit
+ *                          (tests/test_oracle_fast.py), with thresholds at
+ *                          the exact u == p boundary.
  *   or_ref_sir_run      - the reference's own known answers:
  *                          tests/test_simulation.py:95-115 (chain) and
  *                          tests/test_double_buffer.py:247-271 (hierarchy).
@@ -248,6 +253,125 @@ int or_sir_run(const or_model *md, int64_t n_local, int64_t n_total,
         if (trace) memcpy(trace + (size_t)k * (size_t)n_total, cur, (size_t)n_total);
     }
     free(nxt);
+    free(pk);
+    free(halo_slot);
+    return 0;
+}
+
+/*
+ * or_sir_run with the work a CPU implementation would skip taken out; the
+ * results are the same bit for bit (tests/test_oracle_fast.py runs both on
+ * the same inputs).  It is the tuned CPU baseline, next to or_sir_run's
+ * naive one:
+ *   - u < p compares in integers: u = m / 2^23 with m = word & 0x7FFFFF, so
+ *     u < p exactly when m < ceil(p * 2^23) (scaling by 2^23 is exact);
+ *   - an S agent draws first; p_k grows with k, so when m is not below the
+ *     threshold of its whole degree no count of infectious neighbours can
+ *     infect it and its row is not read; otherwise the scan stops at the
+ *     first k whose threshold exceeds m (that neighbour set already holds the
+ *     first infectious neighbour in CSR order, which the history records);
+ *   - states with no spontaneous transition draw nothing;
+ *   - the two state buffers swap instead of being copied each tick.
+ * Same arguments and return value as or_sir_run.
+ */
+int or_sir_run_fast(const or_model *md, int64_t n_local, int64_t n_total,
+                    const int64_t *row_ptr, const int32_t *col_idx,
+                    const int64_t *rng_id, int8_t *state, int32_t *history,
+                    int64_t tick0, int64_t ticks,
+                    const int64_t *ghost_ids, const int64_t *halo_ids,
+                    const int8_t *halo_vals, int64_t n_halo,
+                    int8_t *trace, int nthreads)
+{
+    /* p_k grows with k only for beta in [0, 1]; elsewhere the skip and the
+     * early stop are not valid, and the plain restatement runs instead */
+    if (!(md->beta >= 0.0 && md->beta <= 1.0))
+        return or_sir_run(md, n_local, n_total, row_ptr, col_idx, rng_id, state, history, tick0, ticks,
+                          ghost_ids, halo_ids, halo_vals, n_halo, trace, nthreads);
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    int64_t n_ghost = n_total - n_local;
+    int64_t maxdeg = 0;
+    for (int64_t i = 0; i < n_local; ++i) {
+        int64_t d = row_ptr[i + 1] - row_ptr[i];
+        if (d > maxdeg) maxdeg = d;
+    }
+    double *pk = (double *)malloc(sizeof(double) * (size_t)(maxdeg + 1));
+    uint32_t *thr = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(maxdeg + 1));
+    or_infection_probs(md->beta, maxdeg, pk);
+    for (int64_t k = 0; k <= maxdeg; ++k) {
+        double x = ceil(pk[k] * 8388608.0);
+        thr[k] = x <= 0.0 ? 0u : x >= 8388608.0 ? 8388608u : (uint32_t)x;
+    }
+    thr[0] = 0u;                       /* k = 0 never infects */
+    uint32_t tthr[4];
+    for (int s = 0; s < 4; ++s) {
+        double x = md->trans_p[s] > 0.0 ? ceil(md->trans_p[s] * 8388608.0) : 0.0;
+        tthr[s] = x >= 8388608.0 ? 8388608u : (uint32_t)x;
+    }
+
+    int64_t *halo_slot = NULL;
+    if (n_halo > 0) {
+        int64_t *pairs = (int64_t *)malloc(sizeof(int64_t) * 2 * (size_t)n_ghost);
+        for (int64_t j = 0; j < n_ghost; ++j) { pairs[2*j] = ghost_ids[j]; pairs[2*j+1] = n_local + j; }
+        qsort(pairs, (size_t)n_ghost, 2 * sizeof(int64_t), cmp_pair);
+        halo_slot = (int64_t *)malloc(sizeof(int64_t) * (size_t)n_halo);
+        for (int64_t h = 0; h < n_halo; ++h) {
+            int64_t key[2] = {halo_ids[h], 0};
+            int64_t *hit = (int64_t *)bsearch(key, pairs, (size_t)n_ghost, 2 * sizeof(int64_t), cmp_pair);
+            if (!hit) { free(pairs); free(halo_slot); free(pk); free(thr); return -1; }
+            halo_slot[h] = hit[1];
+        }
+        free(pairs);
+    }
+
+    int8_t *buf = (int8_t *)malloc((size_t)n_total);
+    memcpy(buf, state, (size_t)n_total);
+    int8_t *cur = state, *nxt = buf;
+    const int s_code = md->s_code, inf = md->inf_code, s_target = md->s_target;
+
+    for (int64_t k = 0; k < ticks; ++k) {
+        int64_t t = tick0 + k;
+        if (n_halo > 0) {
+            const int8_t *hv = halo_vals + (size_t)k * (size_t)n_halo;
+            for (int64_t h = 0; h < n_halo; ++h) { cur[halo_slot[h]] = hv[h]; nxt[halo_slot[h]] = hv[h]; }
+        }
+        #pragma omp parallel for schedule(dynamic, 4096)
+        for (int64_t i = 0; i < n_local; ++i) {
+            int s = cur[i];
+            int ns = s;
+            if (s == s_code) {
+                int64_t b = row_ptr[i], e = row_ptr[i + 1];
+                uint32_t m = or_sir_word(md->seed, t, rng_id[i]) & 0x7FFFFFu;
+                if (m < thr[e - b]) {
+                    uint32_t kcnt = 0;
+                    int32_t first = -1;
+                    for (int64_t j = b; j < e; ++j) {
+                        int32_t nb = col_idx[j];
+                        if (cur[nb] == inf) {
+                            if (first < 0) first = nb;
+                            if (m < thr[++kcnt]) {
+                                ns = s_target;
+                                if (history) hist_append(history + (size_t)i * OR_HIST_W, first);
+                                break;
+                            }
+                        }
+                    }
+                }
+            } else if (s >= 0 && s < 4 && tthr[s] != 0u) {
+                if ((or_sir_word(md->seed, t, rng_id[i]) & 0x7FFFFFu) < tthr[s]) ns = md->trans_next[s];
+            }
+            nxt[i] = (int8_t)ns;
+        }
+        if (n_ghost > 0) memcpy(nxt + n_local, cur + n_local, (size_t)n_ghost);
+        int8_t *tmp = cur; cur = nxt; nxt = tmp;
+        if (trace) memcpy(trace + (size_t)k * (size_t)n_total, cur, (size_t)n_total);
+    }
+    if (cur != state) memcpy(state, cur, (size_t)n_total);
+    free(buf);
+    free(thr);
     free(pk);
     free(halo_slot);
     return 0;
