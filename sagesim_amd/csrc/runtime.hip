@@ -531,7 +531,9 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
         }                                                                            \
     } while (0)
     R_HIP(dalloc(ctx, &r, &r.row_ptr, (size_t)nt + 1));
-    R_HIP(dalloc(ctx, &r, &r.col_idx, (size_t)nnz));
+    // whole 16-byte groups: a lane's scan reads its row through aligned 16-byte loads
+    R_HIP(dalloc(ctx, &r, &r.col_idx, (size_t)(nnz + 3) / 4 * 4));
+    if (nnz & 3) R_HIP(hipMemset(r.col_idx + nnz, 0, sizeof(int32_t) * (size_t)(4 - (nnz & 3))));
     // per-agent arrays the light tiles read as 16-agent vectors are padded
     R_HIP(dalloc(ctx, &r, &r.rng_lo, (size_t)pad_agents(nl)));
     R_HIP(dalloc(ctx, &r, &r.deg16, (size_t)pad_agents(nl)));

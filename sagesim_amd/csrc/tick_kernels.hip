@@ -245,24 +245,57 @@ __device__ __forceinline__ int8_t transition(const TickView V, const LCold *C, i
     return draw(V, i) < t ? trans_next(C, s) : s;
 }
 
-// Scan of one row by one lane, eight gathers in flight per round.  Returns
-// whether the count of infectious neighbours reaches the agent's draw.
+// 16-byte global loads and stores (p 16-byte aligned)
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 ld4(const SS_G void *p) {
+    const v4u32 x = *(const SS_G v4u32 *)p;
+    return make_uint4(x.x, x.y, x.z, x.w);
+}
+__device__ __forceinline__ void st4(SS_G void *p, const uint4 v) {
+    v4u32 x;
+    x.x = v.x; x.y = v.y; x.z = v.z; x.w = v.w;
+    *(SS_G v4u32 *)p = x;
+}
+
+// Scan of one row by one lane, eight entries per round.  The entries come
+// from at most three aligned 16-byte loads (col_idx is padded to whole
+// 16-byte groups) and only entries inside the row look up the bitmap, so a
+// round costs the memory pipeline a few lines per lane instead of one per
+// entry.  Returns whether the count of infectious neighbours reaches the
+// agent's draw.
 __device__ __forceinline__ bool scan_lane(const TickView V, const LCold *C, int64_t b, int64_t e, uint32_t m,
                                           int32_t &first, Work &w) {
     const SS_G int32_t *__restrict__ col = V.col_idx;
     const SS_G uint64_t *__restrict__ cb = V.cb;
     int32_t k = 0;
     for (int64_t j = b; j < e; j += 8) {
+        const int64_t a = j & ~(int64_t)3;
+        const int64_t lim = e < j + 8 ? e : j + 8;
+        const uint4 x0 = ld4(col + a);
+        const uint4 x1 = a + 4 < lim ? ld4(col + a + 4) : make_uint4(0u, 0u, 0u, 0u);
+        const uint4 x2 = a + 8 < lim ? ld4(col + a + 8) : make_uint4(0u, 0u, 0u, 0u);
+        // entries j .. j + 7 are words (j & 3) .. (j & 3) + 7 of x0 x1 x2,
+        // selected with named values (an indexed array would go to scratch)
+        const bool s2 = (j & 2) != 0, s1 = (j & 1) != 0;
+        const uint32_t h0 = s2 ? x0.z : x0.x, h1 = s2 ? x0.w : x0.y, h2 = s2 ? x1.x : x0.z;
+        const uint32_t h3 = s2 ? x1.y : x0.w, h4 = s2 ? x1.z : x1.x, h5 = s2 ? x1.w : x1.y;
+        const uint32_t h6 = s2 ? x2.x : x1.z, h7 = s2 ? x2.y : x1.w, h8 = s2 ? x2.z : x2.x;
         int32_t c[8];
+        c[0] = (int32_t)(s1 ? h1 : h0);
+        c[1] = (int32_t)(s1 ? h2 : h1);
+        c[2] = (int32_t)(s1 ? h3 : h2);
+        c[3] = (int32_t)(s1 ? h4 : h3);
+        c[4] = (int32_t)(s1 ? h5 : h4);
+        c[5] = (int32_t)(s1 ? h6 : h5);
+        c[6] = (int32_t)(s1 ? h7 : h6);
+        c[7] = (int32_t)(s1 ? h8 : h7);
         uint64_t wd[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) c[q] = col[j + q < e ? j + q : e - 1];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) wd[q] = cb[c[q] >> 6];
+        for (int q = 0; q < 8; ++q) wd[q] = j + q < e ? cb[c[q] >> 6] : 0ull;
         w.edges += (uint32_t)(e - j < 8 ? e - j : 8);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            if (j + q < e && ((wd[q] >> (c[q] & 63)) & 1ull)) {
+            if ((wd[q] >> (c[q] & 63)) & 1ull) {
                 if (first < 0) first = c[q];
                 ++k;
                 if (m < thr(V, C, k)) return true;
@@ -543,18 +576,6 @@ __device__ __forceinline__ uint32_t eq_bytes4(uint32_t x, uint32_t y) {
     const uint32_t t = x ^ y;
     const uint32_t z = ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t) & 0x80808080u;   // 0x80 where a byte of t is 0
     return (((z >> 7) * 0x00204081u) >> 21) & 0xFu;
-}
-
-// 16-byte global loads and stores (p 16-byte aligned)
-typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ uint4 ld4(const SS_G void *p) {
-    const v4u32 x = *(const SS_G v4u32 *)p;
-    return make_uint4(x.x, x.y, x.z, x.w);
-}
-__device__ __forceinline__ void st4(SS_G void *p, const uint4 v) {
-    v4u32 x;
-    x.x = v.x; x.y = v.y; x.z = v.z; x.w = v.w;
-    *(SS_G v4u32 *)p = x;
 }
 
 __device__ __forceinline__ uint32_t u4_word(const uint4 v, int q) {
