@@ -211,9 +211,12 @@ int ss_halo_pack(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n, int8
 #define SS_NOALLOC_MAX_RECORDS (1 << 20)
 int ss_halo_pack_noalloc(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n, int8_t *out);
 /* Run `ticks` ticks; before tick j the ghosts are refreshed from
- * vals[j * n_bound ...] (bound order), copied on a second stream while the
- * previous tick computes: straight from vals when it lies in memory from
- * ss_host_alloc, else staged through the library's pinned buffers. */
+ * vals[j * n_bound ...] (bound order), copied on a second stream while
+ * earlier ticks compute: straight from vals when it lies in memory from
+ * ss_host_alloc (in groups of up to 32 ticks, each group's copies issued
+ * ahead of its ticks; device staging of up to 2 x 64 MB per replica), else
+ * staged tick by tick through the library's pinned buffers.  vals is
+ * borrowed for the call only. */
 int ss_simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals);
 /* 2-bit halo (no counterpart in the reference, whose halo carries one f32
  * per property and agent, gpu_kernels.py:852-909): the same three calls with

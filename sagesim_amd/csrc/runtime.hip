@@ -129,6 +129,7 @@ void free_replica(ss_replica &r) {
     dfree(r.cnt[0]); dfree(r.cnt[1]); dfree(r.dec); dfree(r.acc);
     dfree(r.map_keys); dfree(r.map_vals); dfree(r.halo_slots);
     for (auto &d : r.halo_dev) dfree(d);
+    for (auto &d : r.halo_grp) dfree(d);
     dfree(r.halo_rec);
     r = ss_replica();
 }
@@ -392,6 +393,11 @@ static hipError_t create_halo_events(ss_ctx *ctx) {
         e = hipEventCreateWithFlags(&ctx->halo_ev[i], hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->halo_used[i], hipEventDisableTiming);
     }
+    for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+        e = hipEventCreateWithFlags(&ctx->grp_done[b], hipEventDisableTiming);
+        for (int i = 0; i < SS_HALO_SPANS && e == hipSuccess; ++i)
+            e = hipEventCreateWithFlags(&ctx->span_ev[b][i], hipEventDisableTiming);
+    }
     return e;
 }
 
@@ -444,6 +450,11 @@ int ss_destroy(ss_ctx *ctx) {
         if (ctx->h_halo_pin[i]) (void)hipHostFree(ctx->h_halo_pin[i]);
         if (ctx->halo_ev[i]) (void)hipEventDestroy(ctx->halo_ev[i]);
         if (ctx->halo_used[i]) (void)hipEventDestroy(ctx->halo_used[i]);
+    }
+    for (int b = 0; b < 2; ++b) {
+        if (ctx->grp_done[b]) (void)hipEventDestroy(ctx->grp_done[b]);
+        for (int i = 0; i < SS_HALO_SPANS; ++i)
+            if (ctx->span_ev[b][i]) (void)hipEventDestroy(ctx->span_ev[b][i]);
     }
     (void)hipStreamDestroy(ctx->stream);
     (void)hipStreamDestroy(ctx->copy_stream);
@@ -1094,6 +1105,8 @@ int ss_halo_bind(ss_ctx *ctx, int32_t rep, const int64_t *ids, int64_t n, int64_
     SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     dfree(r.halo_slots); dfree(r.halo_rec);
     for (auto &d : r.halo_dev) { dfree(d); d = nullptr; }
+    for (auto &d : r.halo_grp) { dfree(d); d = nullptr; }
+    r.halo_grp_bytes = 0;
     r.halo_slots = nullptr; r.halo_rec = nullptr;
     r.n_halo = 0;
     if (n == 0) return SS_OK;
@@ -1207,6 +1220,67 @@ int ss_halo_pack_2bit(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n,
     return halo_pack(ctx, rep, slots, n, out, true);
 }
 
+// One tick of a halo run: the ghost refresh from the staged halo bytes, the
+// bitmap words shared by local agents and ghosts, the tick.
+static hipError_t halo_tick(ss_ctx *ctx, ss_replica &r, int64_t k, const int8_t *staged, bool packed) {
+    const int par = (int)((r.tick + k) & 1);
+    hipError_t e = ss_launch_halo_gather(r.state[0], r.state[1], r.state[par], r.ibits[par], r.halo_rec, staged,
+                                         r.halo_base, r.n_total, r.dev.inf_code, packed, ctx->stream);
+    const int64_t w0 = r.n_local / 64;
+    if (e == hipSuccess && r.halo_base > w0 * 64)
+        e = ss_launch_bits(r.state[par], r.ibits[par], w0, r.halo_base / 64, r.n_total, r.dev.inf_code, ctx->stream);
+    if (e == hipSuccess) e = tick_launch(ctx, k);
+    return e;
+}
+
+// Halo run from page-locked host memory, in groups of at most SS_HALO_GROUP
+// ticks (and SS_HALO_GROUP_BYTES of halo).  A group's halo goes to one of two
+// device buffers in copies issued before any of its kernels, over spans of
+// 1, 1, 2, 4, ... ticks: the first tick waits for one tick's bytes, and the
+// copy engine, faster than the ticks consume, stays ahead after that.  The
+// host issues a tick's kernels without waiting for anything; it waits once a
+// group, for the refreshes of the group before last to have read the buffer
+// it is about to overwrite.
+static int simulate_halo_pinned(ss_ctx *ctx, ss_replica &r, int64_t ticks, const int8_t *vals, int64_t n,
+                                bool packed) {
+    const int64_t gmax = std::max<int64_t>(1, std::min<int64_t>(SS_HALO_GROUP, SS_HALO_GROUP_BYTES / n));
+    const int64_t want = std::min<int64_t>(gmax, ticks) * n;
+    if (r.halo_grp_bytes < want) {
+        SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (auto &d : r.halo_grp) { dfree(d); d = nullptr; }
+        r.halo_grp_bytes = 0;
+        for (auto &d : r.halo_grp) SS_HIP(ctx, dalloc(ctx, &r, &d, (size_t)want));
+        r.halo_grp_bytes = want;
+    }
+    const int64_t per = r.halo_grp_bytes / n;         // ticks a group buffer holds
+    for (int64_t k0 = 0; k0 < ticks;) {
+        const int64_t G = std::min(per, ticks - k0);
+        const int b = (int)(ctx->grp_pos++ & 1);
+        SS_HIP(ctx, hipEventSynchronize(ctx->grp_done[b]));
+        int8_t *buf = r.halo_grp[b];
+        int64_t span_end[SS_HALO_SPANS];
+        int ns = 0;
+        for (int64_t a = 0; a < G; ++ns) {
+            const int64_t e = ns + 1 == SS_HALO_SPANS ? G : std::min(G, a == 0 ? 1 : 2 * a);
+            SS_HIP(ctx, hipMemcpyAsync(buf + a * n, vals + (size_t)(k0 + a) * (size_t)n, (size_t)((e - a) * n),
+                                       hipMemcpyHostToDevice, ctx->copy_stream));
+            SS_HIP(ctx, hipEventRecord(ctx->span_ev[b][ns], ctx->copy_stream));
+            span_end[ns] = e;
+            a = e;
+        }
+        for (int64_t j = 0, s = 0; j < G; ++j) {
+            if (j == (s == 0 ? 0 : span_end[s - 1])) SS_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->span_ev[b][s++], 0));
+            SS_HIP(ctx, halo_tick(ctx, r, k0 + j, buf + j * n, packed));
+        }
+        SS_HIP(ctx, hipEventRecord(ctx->grp_done[b], ctx->stream));
+        k0 += G;
+    }
+    // the caller's buffer is borrowed for this call only
+    SS_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));
+    advance_ticks(ctx, ticks);
+    return SS_OK;
+}
+
 static int simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *vals, bool packed) {
     if (!valid_rep(ctx, rep) || (ticks > 0 && !vals)) SS_FAIL(ctx, SS_ERR_INVALID, "bad replica or NULL");
     if (ticks < 0 || ticks > INT32_MAX) SS_FAIL(ctx, SS_ERR_INVALID, "ticks out of range");
@@ -1225,7 +1299,9 @@ static int simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *
         pinned = e == hipSuccess && at.type == hipMemoryTypeHost;
         if (e != hipSuccess) (void)hipGetLastError();   // pageable memory: not an error here
     }
-    if (!pinned && ctx->h_halo_cap < n) {
+    if (pinned) return simulate_halo_pinned(ctx, r, ticks, vals, n, packed);
+    // pageable memory: staged tick by tick through a ring of page-locked buffers
+    if (ctx->h_halo_cap < n) {
         SS_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));
         for (auto &h : ctx->h_halo_pin) {
             if (h) (void)hipHostFree(h);
@@ -1235,35 +1311,22 @@ static int simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *
         for (auto &h : ctx->h_halo_pin) SS_HIP(ctx, hipHostMalloc((void **)&h, (size_t)n, hipHostMallocDefault));
         ctx->h_halo_cap = n;
     }
-    const int64_t w0 = r.n_local / 64;
     for (int64_t k = 0; k < ticks; ++k) {
         // ring slot b (device buffer, staging buffer, events) is free once the
-        // refresh that last read it has run: the host waits for that, so the
+        // tick that last read it has run: the host waits for that, so the
         // copy stream never waits on the compute stream (a copy queued behind
         // such a wait holds the calling thread until the wait resolves)
         const int b = (int)(ctx->halo_pos++ % SS_HALO_RING);
         SS_HIP(ctx, hipEventSynchronize(ctx->halo_used[b]));
-        const int8_t *src = vals + (size_t)k * (size_t)n;
-        if (!pinned) {
-            std::memcpy(ctx->h_halo_pin[b], src, (size_t)n);
-            src = ctx->h_halo_pin[b];
-        }
+        std::memcpy(ctx->h_halo_pin[b], vals + (size_t)k * (size_t)n, (size_t)n);
         // the copy overlaps the kernels of the ticks before
-        SS_HIP(ctx, hipMemcpyAsync(r.halo_dev[b], src, (size_t)n, hipMemcpyHostToDevice, ctx->copy_stream));
+        SS_HIP(ctx, hipMemcpyAsync(r.halo_dev[b], ctx->h_halo_pin[b], (size_t)n, hipMemcpyHostToDevice,
+                                   ctx->copy_stream));
         SS_HIP(ctx, hipEventRecord(ctx->halo_ev[b], ctx->copy_stream));
         SS_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->halo_ev[b], 0));
-        const int par = (int)((r.tick + k) & 1);
-        SS_HIP(ctx, ss_launch_halo_gather(r.state[0], r.state[1], r.state[par], r.ibits[par], r.halo_rec,
-                                          r.halo_dev[b], r.halo_base, r.n_total, r.dev.inf_code, packed,
-                                          ctx->stream));
+        SS_HIP(ctx, halo_tick(ctx, r, k, r.halo_dev[b], packed));
         SS_HIP(ctx, hipEventRecord(ctx->halo_used[b], ctx->stream));
-        if (r.halo_base > w0 * 64)
-            SS_HIP(ctx, ss_launch_bits(r.state[par], r.ibits[par], w0, r.halo_base / 64, r.n_total, r.dev.inf_code,
-                                       ctx->stream));
-        SS_HIP(ctx, tick_launch(ctx, k));
     }
-    // the caller's buffer is borrowed for this call only
-    if (pinned) SS_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));
     advance_ticks(ctx, ticks);
     return SS_OK;
 }

@@ -14,6 +14,10 @@
 // device slots of a bound halo (and pinned staging buffers): the host runs at
 // most this many ticks ahead of the refreshes that read them
 #define SS_HALO_RING 4
+// page-locked halo input: ticks per group and device bytes per group buffer
+#define SS_HALO_GROUP 32
+#define SS_HALO_GROUP_BYTES (64ll << 20)
+#define SS_HALO_SPANS 8                         // copies per group: spans of 1, 1, 2, 4, ... ticks
 // work counters: S agents visited, draws, scans, CSR entries scanned,
 // counts pushed, replica-ticks taken in the push direction, agents of the
 // tiles that ran
@@ -99,6 +103,8 @@ struct ss_replica {
     int32_t *halo_rec = nullptr;                // record of slot halo_base + g, -1 when not bound
     int64_t halo_base = 0;                      // first slot refreshed (a multiple of 64)
     int8_t *halo_dev[SS_HALO_RING] = {};       // device staging of halo values, a ring by tick
+    int8_t *halo_grp[2] = {};                   // device staging of a group of ticks (page-locked input)
+    int64_t halo_grp_bytes = 0;                 // bytes of each halo_grp buffer
     // bookkeeping
     int64_t tick = 0, tick_init = 0;
     int64_t tick_limit = INT64_MAX;            // replica finishes at this tick
@@ -148,6 +154,9 @@ struct ss_ctx {
     hipEvent_t halo_ev[SS_HALO_RING] = {};      // ring slot b copied to the device
     hipEvent_t halo_used[SS_HALO_RING] = {};    // the refresh that read ring slot b is done
     int64_t halo_pos = 0;                       // ring position of the next halo tick
+    hipEvent_t grp_done[2] = {};                // the last refresh that read group buffer b is done
+    hipEvent_t span_ev[2][SS_HALO_SPANS] = {};  // copy span s of group buffer b is on the device
+    int64_t grp_pos = 0;                        // groups issued (buffer = grp_pos & 1)
     std::string err;
     size_t bytes = 0;
 };

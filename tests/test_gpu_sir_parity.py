@@ -317,6 +317,44 @@ def test_halo_2bit_matches_bytes(eng):
             assert np.array_equal(out[1][1], hist)
 
 
+def test_halo_page_locked_groups(eng):
+    """Page-locked halo input runs in groups of at most 32 ticks, each copied
+    ahead in spans of 1, 1, 2, 4, ... ticks: one 75-tick call (groups of 32,
+    32 and 11 ticks) as 2-bit codes and 40 + 35 ticks as bytes give the
+    oracle's states and history, as does the staged path."""
+    from sagesim_amd import pack_2bit
+    pw = workloads.sbm_partition(n_blocks=8, block_size=3000, deg_in=10, deg_out=2, n_parts=2, part=0,
+                                 ticks=75, seed=29)
+    g = pw.graph
+    n = pw.halo_ids.shape[0]
+    vals = pw.halo_vals(0, pw.ticks)
+    hist = pw.history.copy()
+    want, _ = O.sir_run(O.sir_model(pw.beta, pw.gamma, pw.seed), g.row_ptr, g.col_idx, pw.local_ids, pw.state,
+                        pw.ticks, n_local=g.n, history=hist, ghost_ids=pw.ghost_ids, halo_ids=pw.halo_ids,
+                        halo_vals=vals)
+    for mode in ("2bit-pinned", "bytes-pinned", "bytes-staged"):
+        say("halo groups", mode, g.n, g.n_cols)
+        rep = Replica(eng, g, pw.state, sir_model(pw.beta, pw.gamma), pw.seed, rng_key_id=pw.local_ids,
+                      history=pw.history)
+        eng.map_build(rep.rep, pw.ghost_ids, np.arange(g.n, g.n_cols, dtype=np.int32), 0.5)
+        assert eng.halo_bind(rep.rep, pw.halo_ids) == 0
+        if mode == "2bit-pinned":
+            buf = eng.pinned_empty((pw.ticks, (n + 3) // 4), np.uint8)
+            buf[:] = pack_2bit(vals)
+            eng.simulate_halo_2bit(rep.rep, pw.ticks, buf)
+        else:
+            for t0, t1 in ((0, 40), (40, 75)):
+                if mode == "bytes-pinned":
+                    buf = eng.pinned_empty((t1 - t0, n), np.int8)
+                    buf[:] = vals[t0:t1]
+                else:
+                    buf = vals[t0:t1]
+                eng.simulate_halo(rep.rep, t1 - t0, buf)
+        assert first_diff(rep.states()[:g.n], want[:g.n]) is None, mode
+        assert np.array_equal(rep.history(), hist), mode
+        rep.remove()
+
+
 def test_halo_refresh_large_with_repeats(eng):
     """A halo of 98k records in a random record order, then a binding with
     three ghosts left out and five records repeated at the end with other
