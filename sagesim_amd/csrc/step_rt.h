@@ -161,7 +161,16 @@ struct Prop {
     OwnSlot<T> *own;
     __device__ PropRef<T> operator[](long long i) const {
         const bool mine = i == self;
+#ifdef SS_RT_OFF32
+        // every property array is below 2 GiB (the wrapper's source says so):
+        // a 32-bit byte offset from the array's base, which the loads take as
+        // one register beside the base in scalar registers (an index outside
+        // the array is outside it either way)
+        const unsigned int o = (unsigned int)i * (unsigned int)sizeof(T);
+        return PropRef<T>{(const T *)((const char *)r + o), (T *)((char *)w + o), mine ? own : (OwnSlot<T> *)nullptr};
+#else
         return PropRef<T>{r + i, w + i, mine ? own : (OwnSlot<T> *)nullptr};
+#endif
     }
 };
 

@@ -526,7 +526,12 @@ class Model:
         in registration order, then `array, idx_map` per breed-local array."""
         if groups is None:
             _, groups = self._priority_groups()
-        lines = ['#include "sagesim_rt.h"', ""]
+        lines = []
+        # property arrays below 2 GiB: Prop indexes them with 32-bit byte offsets
+        props = getattr(self, "_host_props", None) or {}
+        if props and all(np.asarray(a).nbytes < (1 << 31) for a in props.values()):
+            lines.append("#define SS_RT_OFF32 1")
+        lines += ['#include "sagesim_rt.h"', ""]
         seen = set()           # one copy of each source text (several functions may share a file)
         for grp in groups:
             for fn in grp.values():

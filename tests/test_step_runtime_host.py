@@ -106,6 +106,7 @@ def test_generated_wrapper_matches_oracle_on_host(tmp_path):
     order, rp, ci, logical, groups = m._host_layout()
     src, names = m.generate_source(groups)
     assert names == ["ss_prio_0"]
+    assert src.startswith("#define SS_RT_OFF32 1")     # 32-bit property offsets (arrays below 2 GiB)
     (tmp_path / "gen.hip").write_text(src)
     rp.astype(np.int64).tofile(tmp_path / "rp.bin")
     ci.astype(np.int32).tofile(tmp_path / "ci.bin")
