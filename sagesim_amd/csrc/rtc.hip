@@ -84,14 +84,19 @@ uint64_t fnv1a(uint64_t h, const void *p, size_t n) {
 // Loops of unknown trip count (a step function's walk over its neighbour row)
 // are unrolled at run time, so one thread has several neighbour loads in
 // flight instead of one round trip per neighbour; SAGESIM_AMD_RTC_UNROLL
-// sets the count (1 disables it).
+// sets the count (1 disables it) and SAGESIM_AMD_RTC_UNROLL_THRESHOLD the
+// size the unrolled body may reach (LLVM lowers the count of a loop whose
+// body would grow past it: a heavy loop body unrolls less).
 std::vector<std::string> rtc_options() {
     std::vector<std::string> o = {"--offload-arch=gfx950", "-O3", "-std=c++20", "-ffp-contract=off",
                                   "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-    int unroll = 16;
+    int unroll = 48;
+    std::string threshold = "1200";
     if (const char *u = std::getenv("SAGESIM_AMD_RTC_UNROLL")) unroll = std::atoi(u);
+    if (const char *t = std::getenv("SAGESIM_AMD_RTC_UNROLL_THRESHOLD")) threshold = t;
     if (unroll > 1) {
         o.insert(o.end(), {"-mllvm", "-unroll-runtime", "-mllvm", "-unroll-count=" + std::to_string(unroll)});
+        if (!threshold.empty()) o.insert(o.end(), {"-mllvm", "-unroll-threshold=" + threshold});
     }
     return o;
 }
