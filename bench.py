@@ -268,7 +268,7 @@ def main():
                     "launch_us": round(launch_us, 2), "push_ticks": push_ticks,
                     "plain_copy_GBs": None if copy_gbs is None else round(copy_gbs, 1),
                     "gather_probe": gather,
-                    "kernel": f"tick_kernel<{args.apl}, false>"}
+                    "kernel": f"tick_kernel<{args.apl}, false, false>"}
         log(f"state-aware {per_launch / 1e6:.2f} MB/launch -> {achieved:.1f} GB/s; touched "
             f"{touched / args.ticks / 1e6:.2f} MB/launch; push ticks {push_ticks}")
 

@@ -130,6 +130,7 @@ void free_replica(ss_replica &r) {
     dfree(r.map_keys); dfree(r.map_vals); dfree(r.halo_slots);
     for (auto &d : r.halo_dev) dfree(d);
     for (auto &d : r.halo_grp) dfree(d);
+    dfree((void *)r.halo_ptrs);
     dfree(r.halo_rec);
     r = ss_replica();
 }
@@ -353,7 +354,7 @@ void advance_ticks(ss_ctx *ctx, int64_t ticks) {
 // launch step k of a tick loop
 hipError_t tick_launch(ss_ctx *ctx, int64_t k) {
     return ss_launch_tick(ctx->d_reps, ctx->d_tiles, ctx->n_tiles, (int)k, ctx->apl, ctx->grid,
-                          ctx->counting ? ctx->d_work : nullptr, ctx->stream);
+                          ctx->counting ? ctx->d_work : nullptr, ctx->halo_fused, ctx->stream);
 }
 
 int launch_loop(ss_ctx *ctx, int64_t ticks) {
@@ -1123,8 +1124,9 @@ int ss_halo_bind(ss_ctx *ctx, int32_t rep, const int64_t *ids, int64_t n, int64_
     for (int32_t s : slots)
         if (s >= 0) lo = std::min<int64_t>(lo, s);
     r.halo_base = lo / 64 * 64;
-    // padded to whole groups of four slots (the refresh reads four records at once)
-    std::vector<int32_t> rec((size_t)((r.n_total - r.halo_base + 3) / 4 * 4), -1);
+    // padded to whole groups of sixteen slots (the refresh reads four records
+    // at once, a lane of the tick kernel's fused refresh sixteen)
+    std::vector<int32_t> rec((size_t)((r.n_total - r.halo_base + 15) / 16 * 16), -1);
     for (int64_t h = 0; h < n; ++h)
         if (slots[(size_t)h] >= 0) rec[(size_t)(slots[(size_t)h] - r.halo_base)] = (int32_t)h;
     SS_HIP(ctx, dalloc(ctx, &r, &r.halo_rec, rec.size()));
@@ -1220,18 +1222,27 @@ int ss_halo_pack_2bit(ss_ctx *ctx, int32_t rep, const int32_t *slots, int64_t n,
     return halo_pack(ctx, rep, slots, n, out, true);
 }
 
-// One tick of a halo run: the ghost refresh from the staged halo bytes, the
-// bitmap words shared by local agents and ghosts, the tick.
-static hipError_t halo_tick(ss_ctx *ctx, ss_replica &r, int64_t k, const int8_t *staged, bool packed) {
+// The ghost refresh of launch step k of a halo run from the staged halo
+// bytes, and the bitmap words shared by local agents and ghosts.
+static hipError_t halo_refresh(ss_ctx *ctx, ss_replica &r, int64_t k, const int8_t *staged, bool packed) {
     const int par = (int)((r.tick + k) & 1);
     hipError_t e = ss_launch_halo_gather(r.state[0], r.state[1], r.state[par], r.ibits[par], r.halo_rec, staged,
                                          r.halo_base, r.n_total, r.dev.inf_code, packed, ctx->stream);
     const int64_t w0 = r.n_local / 64;
     if (e == hipSuccess && r.halo_base > w0 * 64)
         e = ss_launch_bits(r.state[par], r.ibits[par], w0, r.halo_base / 64, r.n_total, r.dev.inf_code, ctx->stream);
+    return e;
+}
+
+// One tick of a halo run: the refresh, then the tick.
+static hipError_t halo_tick(ss_ctx *ctx, ss_replica &r, int64_t k, const int8_t *staged, bool packed) {
+    hipError_t e = halo_refresh(ctx, r, k, staged, packed);
     if (e == hipSuccess) e = tick_launch(ctx, k);
     return e;
 }
+
+static int simulate_halo_groups(ss_ctx *ctx, ss_replica &r, int64_t ticks, const int8_t *vals, int64_t n,
+                                bool packed, int64_t per, bool fuse);
 
 // Halo run from page-locked host memory, in groups of at most SS_HALO_GROUP
 // ticks (and SS_HALO_GROUP_BYTES of halo).  A group's halo goes to one of two
@@ -1239,8 +1250,12 @@ static hipError_t halo_tick(ss_ctx *ctx, ss_replica &r, int64_t k, const int8_t 
 // 1, 1, 2, 4, ... ticks: the first tick waits for one tick's bytes, and the
 // copy engine, faster than the ticks consume, stays ahead after that.  The
 // host issues a tick's kernels without waiting for anything; it waits once a
-// group, for the refreshes of the group before last to have read the buffer
-// it is about to overwrite.
+// group, for the ticks of the group before last to have read the buffer it is
+// about to overwrite.  The first tick of a group has the refresh kernel; with
+// 16-agent wide tiles (and no ghost slot in a hub or heavy tile), every other
+// tick's refresh is fused into the tick before it: that tick writes the
+// ghosts of the next buffer from the next step's halo (halo_next in the
+// replica descriptor), so a group of G ticks launches one refresh, not G.
 static int simulate_halo_pinned(ss_ctx *ctx, ss_replica &r, int64_t ticks, const int8_t *vals, int64_t n,
                                 bool packed) {
     const int64_t gmax = std::max<int64_t>(1, std::min<int64_t>(SS_HALO_GROUP, SS_HALO_GROUP_BYTES / n));
@@ -1253,6 +1268,50 @@ static int simulate_halo_pinned(ss_ctx *ctx, ss_replica &r, int64_t ticks, const
         r.halo_grp_bytes = want;
     }
     const int64_t per = r.halo_grp_bytes / n;         // ticks a group buffer holds
+    const bool fuse = ctx->apl == 16 && r.dev.h64 <= r.n_local && ticks > 1;
+    std::vector<const uint8_t *> hp;
+    if (fuse) {
+        // the staged halo of every step of the call; null where a group starts
+        hp.assign((size_t)ticks, nullptr);
+        int64_t gp = ctx->grp_pos;
+        for (int64_t k0 = 0; k0 < ticks; k0 += per, ++gp) {
+            const int8_t *buf = r.halo_grp[gp & 1];
+            for (int64_t j = 1; j < std::min(per, ticks - k0); ++j) hp[(size_t)(k0 + j)] = (const uint8_t *)(buf + j * n);
+        }
+        if (r.halo_ptrs_cap < ticks) {
+            SS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            dfree((void *)r.halo_ptrs);
+            r.halo_ptrs = nullptr;
+            r.halo_ptrs_cap = 0;
+            SS_HIP(ctx, dalloc(ctx, &r, &r.halo_ptrs, (size_t)ticks));
+            r.halo_ptrs_cap = ticks;
+        }
+        r.dev.halo_next = r.halo_ptrs;
+        r.dev.halo_n = (int32_t)ticks;
+        r.dev.halo_rec = r.halo_rec;
+        r.dev.halo_base = r.halo_base;
+        r.dev.halo_packed = packed ? 1u : 0u;
+    }
+    int rc = push_descriptors(ctx, ticks);              // the compute stream is idle after it
+    r.dev.halo_next = nullptr;                          // later calls upload the descriptor without them
+    r.dev.halo_n = 0;
+    if (rc) return rc;
+    if (fuse)
+        SS_HIP(ctx, hipMemcpy((void *)r.halo_ptrs, hp.data(), sizeof(const uint8_t *) * hp.size(),
+                              hipMemcpyHostToDevice));
+    ctx->halo_fused = fuse;
+    rc = simulate_halo_groups(ctx, r, ticks, vals, n, packed, per, fuse);
+    ctx->halo_fused = false;
+    if (rc) return rc;
+    // the caller's buffer is borrowed for this call only
+    SS_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));
+    advance_ticks(ctx, ticks);
+    return SS_OK;
+}
+
+// The groups of a page-locked halo run (see simulate_halo_pinned).
+static int simulate_halo_groups(ss_ctx *ctx, ss_replica &r, int64_t ticks, const int8_t *vals, int64_t n,
+                                bool packed, int64_t per, bool fuse) {
     for (int64_t k0 = 0; k0 < ticks;) {
         const int64_t G = std::min(per, ticks - k0);
         const int b = (int)(ctx->grp_pos++ & 1);
@@ -1268,16 +1327,25 @@ static int simulate_halo_pinned(ss_ctx *ctx, ss_replica &r, int64_t ticks, const
             span_end[ns] = e;
             a = e;
         }
-        for (int64_t j = 0, s = 0; j < G; ++j) {
-            if (j == (s == 0 ? 0 : span_end[s - 1])) SS_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->span_ev[b][s++], 0));
-            SS_HIP(ctx, halo_tick(ctx, r, k0 + j, buf + j * n, packed));
+        // the compute stream waits for the spans holding ticks up to x, in order
+        int s = 0;
+        auto need = [&](int64_t x) -> hipError_t {
+            hipError_t he = hipSuccess;
+            while (he == hipSuccess && s < ns && (s == 0 ? 0 : span_end[s - 1]) <= x)
+                he = hipStreamWaitEvent(ctx->stream, ctx->span_ev[b][s++], 0);
+            return he;
+        };
+        for (int64_t j = 0; j < G; ++j) {
+            if (j == 0 || !fuse) {
+                SS_HIP(ctx, need(j));
+                SS_HIP(ctx, halo_refresh(ctx, r, k0 + j, buf + j * n, packed));
+            }
+            if (fuse && j + 1 < G) SS_HIP(ctx, need(j + 1));   // this tick writes the next one's ghosts
+            SS_HIP(ctx, tick_launch(ctx, k0 + j));
         }
         SS_HIP(ctx, hipEventRecord(ctx->grp_done[b], ctx->stream));
         k0 += G;
     }
-    // the caller's buffer is borrowed for this call only
-    SS_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));
-    advance_ticks(ctx, ticks);
     return SS_OK;
 }
 
@@ -1290,7 +1358,6 @@ static int simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *
     if (n == 0) return ss_simulate(ctx, ticks);
     int rc;
     if (ctx->tiles_dirty && (rc = build_tiles(ctx))) return rc;
-    if ((rc = push_descriptors(ctx, ticks))) return rc;
     // values in page-locked memory are copied from where they are
     bool pinned = false;
     {
@@ -1300,6 +1367,7 @@ static int simulate_halo(ss_ctx *ctx, int32_t rep, int64_t ticks, const int8_t *
         if (e != hipSuccess) (void)hipGetLastError();   // pageable memory: not an error here
     }
     if (pinned) return simulate_halo_pinned(ctx, r, ticks, vals, n, packed);
+    if ((rc = push_descriptors(ctx, ticks))) return rc;
     // pageable memory: staged tick by tick through a ring of page-locked buffers
     if (ctx->h_halo_cap < n) {
         SS_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));

@@ -59,6 +59,17 @@ struct ss_rep_dev {
     uint8_t *tact[2];
     int64_t n_hub, h64;
     int32_t hub0, heavy0, light0, light_cap;
+    // Fused halo refresh (a run fed by a host halo): halo_next[k] is the
+    // device copy of the halo of launch step k of the current call, or null
+    // where step k starts with its own refresh; step k - 1 then writes that
+    // halo into the ghost slots of the next buffer (and their bitmap bits) as
+    // the refresh kernel would before step k.  halo_rec[g] is the record of
+    // slot halo_base + g (-1: not bound), padded to whole 16-slot groups.
+    const uint8_t *const *halo_next;
+    const int32_t *halo_rec;
+    int64_t halo_base;
+    int32_t halo_n;             // entries of halo_next (0: no fused refresh)
+    uint32_t halo_packed;       // 2-bit codes, four records a byte
 };
 
 #define SS_LTAG_SHIFT 10

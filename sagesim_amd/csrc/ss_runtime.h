@@ -25,7 +25,7 @@
 
 // kernel launchers (tick_kernels.hip)
 hipError_t ss_launch_tick(const ss_rep_dev *reps, const ss_tile *tiles, int n_tiles, int k, int apl, int grid,
-                          unsigned long long *counters, hipStream_t st);
+                          unsigned long long *counters, bool halo, hipStream_t st);
 // agents of one HEAVY tile (4 wavefronts x 4 agents, tick_kernels.hip SS_HPW)
 #define SS_HEAVY_TILE 16
 #define SS_HEAVY_SHIFT 4
@@ -105,6 +105,8 @@ struct ss_replica {
     int8_t *halo_dev[SS_HALO_RING] = {};       // device staging of halo values, a ring by tick
     int8_t *halo_grp[2] = {};                   // device staging of a group of ticks (page-locked input)
     int64_t halo_grp_bytes = 0;                 // bytes of each halo_grp buffer
+    const uint8_t **halo_ptrs = nullptr;        // per launch step of a call: its staged halo (fused refresh)
+    int64_t halo_ptrs_cap = 0;
     // bookkeeping
     int64_t tick = 0, tick_init = 0;
     int64_t tick_limit = INT64_MAX;            // replica finishes at this tick
@@ -157,6 +159,7 @@ struct ss_ctx {
     hipEvent_t grp_done[2] = {};                // the last refresh that read group buffer b is done
     hipEvent_t span_ev[2][SS_HALO_SPANS] = {};  // copy span s of group buffer b is on the device
     int64_t grp_pos = 0;                        // groups issued (buffer = grp_pos & 1)
+    bool halo_fused = false;                    // this call's ticks fuse the next tick's halo refresh
     std::string err;
     size_t bytes = 0;
 };

@@ -75,6 +75,10 @@ struct Cold {
     uint8_t ptag;                // tile tag of the next tick
     const int8_t *pcur;          // states of this tick when counts go only to susceptible agents, else null
     int8_t s_code;
+    const uint8_t *hnext;        // fused halo refresh: the next step's halo, or null
+    const int32_t *hrec;         // record of slot hbase + g
+    int64_t hbase;
+    uint32_t hpacked;            // 2-bit codes
 };
 
 typedef SS_L Cold LCold;        // the record as it sits in LDS
@@ -587,6 +591,7 @@ __device__ __forceinline__ int64_t deg_of(const TickView &V, int64_t i, uint32_t
     return d16 == 65535u ? V.row_ptr[i + 1] - V.row_ptr[i] : (int64_t)d16;
 }
 
+template <bool HALO>
 __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int64_t begin, int64_t end, Work &w,
                                            Dir &dir, WideQueue &Q) {
     const int lane = opaque((int)threadIdx.x & 63), wv = threadIdx.x >> 6;
@@ -786,6 +791,36 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
             nsv.z = (r.z & m2) | (sv.z & ~m2);
             nsv.w = (r.w & m3) | (sv.w & ~m3);
         }
+    }
+    // fused halo refresh: the lane's ghost slots take the next step's halo
+    // (what the refresh kernel would write before that step); a lane with
+    // ghosts starts at or after hbase (a multiple of 64, at most n_local).
+    // Compiled only into the kernels that halo runs launch.
+    if (HALO && nl < nv && C->hnext != nullptr) {
+        const SS_G int32_t *rr = (const SS_G int32_t *)C->hrec + (i0 - C->hbase);
+        const SS_G uint8_t *hv = (const SS_G uint8_t *)C->hnext;
+        const bool pk = C->hpacked != 0u;
+        int32_t rc[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint4 x = ld4(rr + 4 * q);
+            rc[4 * q] = (int32_t)x.x; rc[4 * q + 1] = (int32_t)x.y; rc[4 * q + 2] = (int32_t)x.z; rc[4 * q + 3] = (int32_t)x.w;
+        }
+        uint32_t gv[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            gv[j] = 0u;
+            if (j >= nl && j < nv && rc[j] >= 0) gv[j] = hv[pk ? (rc[j] >> 2) : rc[j]];
+        }
+        uint32_t wv[4] = {nsv.x, nsv.y, nsv.z, nsv.w};
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (j >= nl && j < nv && rc[j] >= 0) {
+                const uint32_t c = pk ? (gv[j] >> (2 * (rc[j] & 3))) & 3u : gv[j] & 0xFFu;
+                wv[j >> 2] = (wv[j >> 2] & ~(0xFFu << (8 * (j & 3)))) | c << (8 * (j & 3));
+            }
+        }
+        nsv = make_uint4(wv[0], wv[1], wv[2], wv[3]);
     }
     if (nv == 16) {
         st4(V.nxt + i0, nsv);
@@ -1092,7 +1127,7 @@ template <> struct LightQueue<SS_WIDE_A> { typedef WideQueue type; };
 // at three waves.
 // CNT: the work counters are kept (a separate instantiation, so that the
 // counting registers cost nothing when counting is off).
-template <int A, bool CNT>
+template <int A, bool CNT, bool HALO = false>
 __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void tick_kernel(const ss_rep_dev *__restrict__ reps,
                                                         const ss_tile *__restrict__ tiles, int n_tiles, int k,
                                                         unsigned long long *counters) {
@@ -1160,6 +1195,12 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
                     C->ptag = ss_front_tag(t1);
                     C->pcur = R->push_sus_only ? (const int8_t *)((t & 1) != 0 ? R->state[1] : R->state[0]) : nullptr;
                     C->s_code = R->s_code;
+                    if constexpr (HALO) {
+                        C->hnext = k + 1 < R->halo_n ? R->halo_next[k + 1] : nullptr;
+                        C->hrec = R->halo_rec;
+                        C->hbase = R->halo_base;
+                        C->hpacked = R->halo_packed;
+                    }
                 }
                 if (threadIdx.x < 4) {
                     C->tT[threadIdx.x] = R->trans_T[threadIdx.x];
@@ -1218,7 +1259,7 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
         if (threadIdx.x == 0) w.swept += (uint32_t)(end - begin);
         if (kind == SS_TILE_LIGHT) {
             if constexpr (A == SS_WIDE_A)
-                light_wide(V, C, begin, end, w, dir, Q);
+                light_wide<HALO>(V, C, begin, end, w, dir, Q);
             else
                 light_tile<A>(V, C, begin, end, tag_lo, tag_hi, w, dir, Q);
         }
@@ -1311,8 +1352,18 @@ inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b);
 
 // ---- launchers (host) ----
 hipError_t ss_launch_tick(const ss_rep_dev *reps, const ss_tile *tiles, int n_tiles, int k, int apl, int grid,
-                          unsigned long long *counters, hipStream_t st) {
+                          unsigned long long *counters, bool halo, hipStream_t st) {
     if (n_tiles <= 0) return hipSuccess;
+    if (halo) {                                  // fused halo refresh: 16-agent wide tiles only
+        if (apl != 16) return hipErrorInvalidValue;
+        if (counters)
+            hipLaunchKernelGGL((tick_kernel<16, true, true>), dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles, n_tiles,
+                               k, counters);
+        else
+            hipLaunchKernelGGL((tick_kernel<16, false, true>), dim3(grid), dim3(SS_BLOCK), 0, st, reps, tiles,
+                               n_tiles, k, counters);
+        return hipGetLastError();
+    }
 #define SS_TICK_CASE(a)                                                                                     \
     case a:                                                                                                 \
         if (counters)                                                                                       \
