@@ -1268,6 +1268,14 @@ static int simulate_halo_pinned(ss_ctx *ctx, ss_replica &r, int64_t ticks, const
         r.halo_grp_bytes = want;
     }
     const int64_t per = r.halo_grp_bytes / n;         // ticks a group buffer holds
+    // the call's first tick is copied before the rest of its set-up, which
+    // its transfer then overlaps
+    if (ticks > 0) {
+        const int b0 = (int)(ctx->grp_pos & 1);
+        SS_HIP(ctx, hipEventSynchronize(ctx->grp_done[b0]));
+        SS_HIP(ctx, hipMemcpyAsync(r.halo_grp[b0], vals, (size_t)n, hipMemcpyHostToDevice, ctx->copy_stream));
+        SS_HIP(ctx, hipEventRecord(ctx->span_ev[b0][0], ctx->copy_stream));
+    }
     const bool fuse = ctx->apl == 16 && r.dev.h64 <= r.n_local && ticks > 1;
     std::vector<const uint8_t *> hp;
     if (fuse) {
@@ -1315,16 +1323,21 @@ static int simulate_halo_groups(ss_ctx *ctx, ss_replica &r, int64_t ticks, const
     for (int64_t k0 = 0; k0 < ticks;) {
         const int64_t G = std::min(per, ticks - k0);
         const int b = (int)(ctx->grp_pos++ & 1);
-        SS_HIP(ctx, hipEventSynchronize(ctx->grp_done[b]));
         int8_t *buf = r.halo_grp[b];
         int64_t span_end[SS_HALO_SPANS];
         int ns = 0;
-        for (int64_t a = 0; a < G; ++ns) {
+        int64_t a = 0;
+        if (k0 == 0) {                                   // the first span went ahead (simulate_halo_pinned)
+            span_end[ns++] = a = 1;
+        } else {
+            SS_HIP(ctx, hipEventSynchronize(ctx->grp_done[b]));
+        }
+        while (a < G) {
             const int64_t e = ns + 1 == SS_HALO_SPANS ? G : std::min(G, a == 0 ? 1 : 2 * a);
             SS_HIP(ctx, hipMemcpyAsync(buf + a * n, vals + (size_t)(k0 + a) * (size_t)n, (size_t)((e - a) * n),
                                        hipMemcpyHostToDevice, ctx->copy_stream));
             SS_HIP(ctx, hipEventRecord(ctx->span_ev[b][ns], ctx->copy_stream));
-            span_end[ns] = e;
+            span_end[ns++] = e;
             a = e;
         }
         // the compute stream waits for the spans holding ticks up to x, in order
