@@ -1268,6 +1268,12 @@ static int simulate_halo_pinned(ss_ctx *ctx, ss_replica &r, int64_t ticks, const
         r.halo_grp_bytes = want;
     }
     const int64_t per = r.halo_grp_bytes / n;         // ticks a group buffer holds
+    // the caller's buffer is borrowed for this call only: every way out of it,
+    // errors included, waits for the copies from it
+    struct CopyWait {
+        hipStream_t s;
+        ~CopyWait() { (void)hipStreamSynchronize(s); }
+    } copy_wait{ctx->copy_stream};
     // the call's first tick is copied before the rest of its set-up, which
     // its transfer then overlaps
     if (ticks > 0) {
@@ -1311,7 +1317,6 @@ static int simulate_halo_pinned(ss_ctx *ctx, ss_replica &r, int64_t ticks, const
     rc = simulate_halo_groups(ctx, r, ticks, vals, n, packed, per, fuse);
     ctx->halo_fused = false;
     if (rc) return rc;
-    // the caller's buffer is borrowed for this call only
     SS_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));
     advance_ticks(ctx, ticks);
     return SS_OK;
