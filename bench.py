@@ -141,8 +141,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path (a partition's share of a "
+              "sharded run is the partition leg: ghosts through the hash map, a host halo every tick).",
+              file=sys.stderr)
         sys.exit(2)
 
     from sagesim_amd import Engine, Replica, workloads
