@@ -317,12 +317,14 @@ def test_halo_2bit_matches_bytes(eng):
             assert np.array_equal(out[1][1], hist)
 
 
-def test_halo_page_locked_groups(eng):
+@pytest.mark.parametrize("apl", [16, 4])
+def test_halo_page_locked_groups(eng, apl):
     """Page-locked halo input runs in groups of at most 32 ticks, each copied
     ahead in spans of 1, 1, 2, 4, ... ticks: one 75-tick call (groups of 32,
     32 and 11 ticks) as 2-bit codes and 40 + 35 ticks as bytes give the
-    oracle's states and history, as does the staged path."""
-    from sagesim_amd import pack_2bit
+    oracle's states and history, as does the staged path.  With 16 agents a
+    lane the ticks inside a group carry the next tick's refresh; with 4 the
+    refresh kernel runs before every tick."""
     pw = workloads.sbm_partition(n_blocks=8, block_size=3000, deg_in=10, deg_out=2, n_parts=2, part=0,
                                  ticks=75, seed=29)
     g = pw.graph
@@ -332,8 +334,17 @@ def test_halo_page_locked_groups(eng):
     want, _ = O.sir_run(O.sir_model(pw.beta, pw.gamma, pw.seed), g.row_ptr, g.col_idx, pw.local_ids, pw.state,
                         pw.ticks, n_local=g.n, history=hist, ghost_ids=pw.ghost_ids, halo_ids=pw.halo_ids,
                         halo_vals=vals)
+    eng.set_tiling(agents_per_lane=apl)
+    try:
+        _halo_groups_modes(eng, pw, g, n, vals, want, hist, apl)
+    finally:
+        eng.set_tiling()
+
+
+def _halo_groups_modes(eng, pw, g, n, vals, want, hist, apl):
+    from sagesim_amd import pack_2bit
     for mode in ("2bit-pinned", "bytes-pinned", "bytes-staged"):
-        say("halo groups", mode, g.n, g.n_cols)
+        say("halo groups", mode, "agents per lane", apl, g.n, g.n_cols)
         rep = Replica(eng, g, pw.state, sir_model(pw.beta, pw.gamma), pw.seed, rng_key_id=pw.local_ids,
                       history=pw.history)
         eng.map_build(rep.rep, pw.ghost_ids, np.arange(g.n, g.n_cols, dtype=np.int32), 0.5)
