@@ -125,6 +125,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=50.0, help="serving leg: arrivals per second")
     ap.add_argument("--serve-capacity", type=int, default=200,
                     help="serving leg: requests released at once to measure capacity (0 skips)")
+    ap.add_argument("--partition-chunk", type=int, default=10,
+                    help="partition leg: ticks per ss_simulate_halo_2bit call (1: a call per tick, as a live "
+                         "exchange with peers would make)")
     ap.add_argument("--partition-ticks", type=int, default=150,
                     help="partition leg (one SBM part with ghosts and a halo feed): ticks (0 skips)")
     ap.add_argument("--partition-deg-out", default="2.0,0.17",
@@ -454,7 +457,7 @@ def partition_leg(eng, args, deg_out=2.0):
     log(f"partition: {g.n} local agents, {n_ghost} ghosts, {g.nnz} entries, built in {t_gen:.1f}s, "
         f"uploaded in {t_up:.1f}s")
     disc = ghost_discovery(eng, g, pw.local_ids)
-    chunk = 10
+    chunk = max(1, args.partition_chunk)
     t_run = 0.0
     exact = None
     n_halo = pw.halo_ids.shape[0]
@@ -498,7 +501,8 @@ def partition_leg(eng, args, deg_out=2.0):
     return {"workload": f"SIR on SBM part 0 of 8 (64 blocks x 500k agents, mean degree 12 in, {deg_out:g} out), "
                         "ghost ids through the hash map, host halo every tick as 2-bit codes, history 32",
             "local_agents": int(g.n), "ghosts": int(n_ghost), "nnz": int(g.nnz), "ticks": pw.ticks,
-            "halo_ids_missing": int(miss), "seconds": round(t_run, 4), "ms_per_tick": round(t_run * 1e3 / pw.ticks, 4),
+            "halo_ids_missing": int(miss), "ticks_per_call": chunk, "seconds": round(t_run, 4),
+            "ms_per_tick": round(t_run * 1e3 / pw.ticks, 4),
             "local_agent_ticks_per_s": rate, "halo_records": int(n_halo),
             "halo_bytes_per_tick": int((n_halo + 3) // 4), "halo_format": "2-bit state codes, four records a byte",
             "oracle_window_ticks": chunk, "oracle_window_exact": exact, "oracle_whole_run_exact": whole,
