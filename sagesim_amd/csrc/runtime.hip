@@ -122,7 +122,7 @@ int apply_model(ss_ctx *ctx, ss_replica &r, const ss_model *m) {
 }
 
 void free_replica(ss_replica &r) {
-    dfree(r.row_ptr); dfree(r.col_idx); dfree(r.rng_lo); dfree(r.rng_hi); dfree(r.deg16);
+    dfree(r.row_ptr); dfree(r.col_idx); dfree(r.rng_lo); dfree(r.rng_hi); dfree(r.deg8);
     dfree(r.state[0]); dfree(r.state[1]); dfree(r.init_state);
     dfree(r.ibits[0]); dfree(r.ibits[1]);
     dfree(r.history); dfree(r.init_history); dfree(r.inf_T);
@@ -548,7 +548,7 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
     if (nnz & 3) R_HIP(hipMemset(r.col_idx + nnz, 0, sizeof(int32_t) * (size_t)(4 - (nnz & 3))));
     // per-agent arrays the light tiles read as 16-agent vectors are padded
     R_HIP(dalloc(ctx, &r, &r.rng_lo, (size_t)pad_agents(nl)));
-    R_HIP(dalloc(ctx, &r, &r.deg16, (size_t)pad_agents(nl)));
+    R_HIP(dalloc(ctx, &r, &r.deg8, (size_t)pad_agents(nl)));
     if (need_hi) R_HIP(dalloc(ctx, &r, &r.rng_hi, (size_t)nl));
     for (int b = 0; b < 2; ++b) {
         R_HIP(dalloc(ctx, &r, &r.state[b], (size_t)pad_agents(nt)));
@@ -600,10 +600,10 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
         }
         r.key_hash = kh;
         if (nl) R_HIP(hipMemcpy(r.rng_lo, lo.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice));
-        std::vector<uint16_t> d16((size_t)std::max<int64_t>(nl, 1));
+        std::vector<uint8_t> d8((size_t)std::max<int64_t>(nl, 1));
         for (int64_t i = 0; i < nl; ++i)
-            d16[i] = (uint16_t)std::min<int64_t>(sp->row_ptr[i + 1] - sp->row_ptr[i], 65535);
-        if (nl) R_HIP(hipMemcpy(r.deg16, d16.data(), sizeof(uint16_t) * nl, hipMemcpyHostToDevice));
+            d8[i] = (uint8_t)std::min<int64_t>(sp->row_ptr[i + 1] - sp->row_ptr[i], 255);
+        if (nl) R_HIP(hipMemcpy(r.deg8, d8.data(), nl, hipMemcpyHostToDevice));
         if (need_hi) R_HIP(hipMemcpy(r.rng_hi, hi.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice));
     }
     if (nt) R_HIP(hipMemcpy(r.init_state, sp->state, (size_t)nt, hipMemcpyHostToDevice));
@@ -627,7 +627,7 @@ int ss_replica_create(ss_ctx *ctx, const ss_replica_spec *sp, int32_t *rep_out) 
     r.dev.col_idx = r.col_idx;
     r.dev.rng_lo = r.rng_lo;
     r.dev.rng_hi = r.rng_hi;
-    r.dev.deg16 = r.deg16;
+    r.dev.deg8 = r.deg8;
     r.dev.state[0] = r.state[0];
     r.dev.state[1] = r.state[1];
     r.dev.ibits[0] = r.ibits[0];

@@ -32,7 +32,7 @@ struct ss_rep_dev {
     int64_t tick0;              // tick of launch step k = 0
     int64_t tick_end;           // ticks >= tick_end are not run (replica finished)
     int64_t n_local, n_total;
-    const uint16_t *deg16;      // [n_local] min(degree, 65535); 65535 means "read row_ptr"
+    const uint8_t *deg8;        // [n_local] min(degree, 255); 255 means "read row_ptr"
     // Push direction.  cnt[t & 1][i] = number of CSR entries of agent i that
     // are infectious at tick t, pushed by tick t - 1 along the rows of the
     // agents it leaves infectious (the CSR is symmetric) when it decides to,

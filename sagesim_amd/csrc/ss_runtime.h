@@ -83,7 +83,7 @@ struct ss_replica {
     int64_t *row_ptr = nullptr;
     int32_t *col_idx = nullptr;
     uint32_t *rng_lo = nullptr, *rng_hi = nullptr;
-    uint16_t *deg16 = nullptr;
+    uint8_t *deg8 = nullptr;
     int8_t *state[2] = {nullptr, nullptr};
     int8_t *init_state = nullptr;
     uint64_t *ibits[2] = {nullptr, nullptr};

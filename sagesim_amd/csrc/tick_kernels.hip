@@ -49,7 +49,7 @@ struct TickView {
     SS_G int8_t *nxt;
     const SS_G uint64_t *cb;
     SS_G uint64_t *nb;
-    const SS_G uint16_t *deg16;
+    const SS_G uint8_t *deg8;
     SS_G uint16_t *cnt;          // infectious-neighbour counts of this tick, null when this tick pulls
     int32_t last;
     int8_t s_code, inf_code, s_target;
@@ -438,7 +438,7 @@ __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int
         if (i < end && i < V.n_local) {
             const bool sc = (cand >> a) & 1u;
             const bool tr = trans_thr(C, s[a]) != 0u;
-            if (sc || tr || s[a] == V.inf_code) deg[a] = V.deg16[i];
+            if (sc || tr || s[a] == V.inf_code) deg[a] = V.deg8[i];
             if (sc || tr) {
                 key[a] = V.rng_lo[i];
                 if (V.rng_hi) khi[a] = V.rng_hi[i];
@@ -461,14 +461,14 @@ __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int
                     if (V.cnt) {
                         if (m < thr(V, C, (int64_t)kc[a])) {
                             ns[a] = V.s_target;
-                            dir_change(dir, V, s[a], ns[a], deg[a] == 65535u ? (int64_t)(V.row_ptr[i + 1] - V.row_ptr[i])
+                            dir_change(dir, V, s[a], ns[a], deg[a] == 255u ? (int64_t)(V.row_ptr[i + 1] - V.row_ptr[i])
                                                                              : (int64_t)deg[a]);
                             // the history's first infectious neighbour (m = 0 stops the scan there)
                             enq = hist = C->history != nullptr;
                             m = 0u;
                         }
-                    } else if (m < thr(V, C, deg[a] == 65535u ? (int64_t)V.last : (int64_t)deg[a])) {
-                        // a capped degree (65535) bounds with the last table entry
+                    } else if (m < thr(V, C, deg[a] == 255u ? (int64_t)V.last : (int64_t)deg[a])) {
+                        // a capped degree (255) bounds with the last table entry
                         enq = true;
                         scanned |= 1u << a;
                         ++w.scans;
@@ -480,7 +480,8 @@ __device__ __forceinline__ void light_tile(const TickView V, const LCold *C, int
                     ++w.draws;
                     if (draw_key(V, key[a], khi[a]) < t) ns[a] = trans_next(C, s[a]);
                 }
-                dir_change(dir, V, s[a], ns[a], deg[a]);
+                dir_change(dir, V, s[a], ns[a], deg[a] == 255u ? (int64_t)(V.row_ptr[i + 1] - V.row_ptr[i])
+                                                               : (int64_t)deg[a]);
             }
         }
         const uint64_t qm = __ballot(enq);
@@ -564,8 +565,8 @@ __device__ __forceinline__ void put_bits(SS_G uint64_t *nb, int64_t word, uint64
 #define SS_WIDE_SHORT 512        // rows longer than this are scanned by the whole wavefront
 struct alignas(16) WideQueue {
     // draw entry: x = RNG key (pull) or infectious-neighbour count (push);
-    //   y = slot | transition << 10 | (state & 3) << 11 | deg16 << 16 (deg16 only on pull)
-    // scan entry: x = draw; y = slot | history-only << 10 | deg16 << 16
+    //   y = slot | transition << 10 | (state & 3) << 11 | deg8 << 16 (deg8 only on pull)
+    // scan entry: x = draw; y = slot | history-only << 10 | deg8 << 16
     uint2 e[SS_BLOCK / 64][SS_WIDE_CHUNK];
     uint8_t res[SS_BLOCK / 64][SS_WIDE_CHUNK];   // next state by chunk slot (queued agents only)
 };
@@ -586,9 +587,9 @@ __device__ __forceinline__ uint32_t u4_word(const uint4 v, int q) {
     return q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w;
 }
 
-// Degree of local agent i from its 16-bit copy (65535: read the row bounds).
-__device__ __forceinline__ int64_t deg_of(const TickView &V, int64_t i, uint32_t d16) {
-    return d16 == 65535u ? V.row_ptr[i + 1] - V.row_ptr[i] : (int64_t)d16;
+// Degree of local agent i from its 8-bit copy (255: read the row bounds).
+__device__ __forceinline__ int64_t deg_of(const TickView &V, int64_t i, uint32_t d8) {
+    return d8 == 255u ? V.row_ptr[i + 1] - V.row_ptr[i] : (int64_t)d8;
 }
 
 template <bool HALO>
@@ -616,11 +617,11 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
     uint4 sv = make_uint4(0u, 0u, 0u, 0u);
     if (nv > 0) sv = ld4(V.cur + i0);
     uint32_t a[16];
-    uint32_t dg[8];
+    uint32_t dg[4];
 #pragma unroll
     for (int q = 0; q < 16; ++q) a[q] = 0u;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) dg[q] = 0u;
+    for (int q = 0; q < 4; ++q) dg[q] = 0u;
     if (nl > 0) {
         if (!pull) {
 #pragma unroll
@@ -636,10 +637,9 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
                 const uint4 x = ld4(V.rng_lo + i0 + 4 * q);
                 a[4 * q] = x.x; a[4 * q + 1] = x.y; a[4 * q + 2] = x.z; a[4 * q + 3] = x.w;
             }
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const uint4 x = ld4(V.deg16 + i0 + 8 * q);
-                dg[4 * q] = x.x; dg[4 * q + 1] = x.y; dg[4 * q + 2] = x.z; dg[4 * q + 3] = x.w;
+            {
+                const uint4 x = ld4(V.deg8 + i0);
+                dg[0] = x.x; dg[1] = x.y; dg[2] = x.z; dg[3] = x.w;
             }
         }
     }
@@ -664,7 +664,7 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
         const bool is_s = (int8_t)s == V.s_code;
         smask |= (is_s ? 1u : 0u) << j;
         tmask |= (!is_s && s < 4u && ((tbits >> s) & 1u) ? 1u : 0u) << j;
-        const uint32_t nz = pull ? ((dg[j >> 1] >> (16 * (j & 1))) & 0xFFFFu) : a[j];
+        const uint32_t nz = pull ? ((dg[j >> 2] >> (8 * (j & 3))) & 0xFFu) : a[j];
         cmask |= (nz != 0u ? 1u : 0u) << j;
     }
     smask &= lmask;
@@ -685,7 +685,7 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
         for (int j = 0; j < 16; ++j) {
             if ((enq >> j) & 1u) {
                 const uint32_t s = (u4_word(sv, j >> 2) >> (8 * (j & 3))) & 3u;
-                const uint32_t d = pull ? ((dg[j >> 1] >> (16 * (j & 1))) & 0xFFFFu) : 0u;
+                const uint32_t d = pull ? ((dg[j >> 2] >> (8 * (j & 3))) & 0xFFu) : 0u;
                 E[pos] = make_uint2(a[j], (uint32_t)(16 * lane + j) | ((tmask >> j) & 1u) << 10 | s << 11 | d << 16);
                 ++pos;
             }
@@ -722,13 +722,13 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
                             if (C->history) {
                                 // the history's first infectious neighbour (draw 0 stops the scan there)
                                 sc = true;
-                                se = make_uint2(0u, slot | 1u << 10 | (uint32_t)V.deg16[i] << 16);
+                                se = make_uint2(0u, slot | 1u << 10 | (uint32_t)V.deg8[i] << 16);
                             }
                         }
                     } else {
                         const uint32_t d = en.y >> 16;
-                        // a capped degree (65535) bounds with the last table entry
-                        if (m < thr(V, C, d == 65535u ? (int64_t)V.last : (int64_t)d)) {
+                        // a capped degree (255) bounds with the last table entry
+                        if (m < thr(V, C, d == 255u ? (int64_t)V.last : (int64_t)d)) {
                             sc = true;
                             se = make_uint2(m, slot | d << 16);
                             ++w.scans;
@@ -736,7 +736,7 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
                     }
                 }
                 if (ns != s)
-                    dir_change(dir, V, s, ns, deg_of(V, i, pull ? en.y >> 16 : (uint32_t)V.deg16[i]));
+                    dir_change(dir, V, s, ns, deg_of(V, i, pull ? en.y >> 16 : (uint32_t)V.deg8[i]));
                 busy |= ns != s || ns == V.inf_code || trans_thr(C, ns) != 0u;
                 res[slot] = (uint8_t)ns;
             }
@@ -759,7 +759,7 @@ __device__ __forceinline__ void light_wide(const TickView V, const LCold *C, int
                 const int64_t i = c0 + (y & 1023u);
                 const uint32_t d = y >> 16;
                 b = V.row_ptr[i];
-                e = d == 65535u ? V.row_ptr[i + 1] : b + (int64_t)d;
+                e = d == 255u ? V.row_ptr[i + 1] : b + (int64_t)d;
                 lng = e - b > SS_WIDE_SHORT;
             }
             bool inf = false;
@@ -1216,7 +1216,7 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
                 V.nxt = (SS_G int8_t *)(odd ? R->state[0] : R->state[1]);
                 V.cb = (const SS_G uint64_t *)(odd ? R->ibits[1] : R->ibits[0]);
                 V.nb = (SS_G uint64_t *)(odd ? R->ibits[0] : R->ibits[1]);
-                V.deg16 = (const SS_G uint16_t *)R->deg16;
+                V.deg8 = (const SS_G uint8_t *)R->deg8;
                 // this tick writes the next tick's tags when the rule says so
                 V.cnt = dec == 0u ? nullptr : (SS_G uint16_t *)R->cnt[t & 1];
                 V.push_next = produce ? 1 : 0;

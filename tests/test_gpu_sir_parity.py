@@ -78,8 +78,11 @@ def test_er_sir_every_tick(eng):
                                                       (1 << 30, 1 << 30, 8, True), (1024, 64, 16, True),
                                                       (1 << 30, 1 << 30, 16, True), (8, 8, 16, False)])
 def test_ba_launch_shapes(eng, hub, heavy, apl, graph_loop):
-    """BA(50k, 5), 60 ticks, in one captured loop: every launch shape gives the oracle's states."""
+    """BA(50k, 5), 60 ticks, in one captured loop: every launch shape gives the oracle's states.
+    With hub and heavy at 2^30 every row is light, so rows of degree 255 and more (whose
+    one-byte degree is the "read the row bounds" mark) go through the light paths."""
     w = workloads.ba_sir(n=50_000, m=5, beta=0.06, ticks=60, seed=3)
+    assert int(w.graph.degrees().max()) > 255
     say("ba launch shape", hub, heavy, apl, graph_loop)
     eng.set_tiling(heavy, apl, 0, graph_loop, hub_degree=hub)
     rep = Replica(eng, w.graph, w.state, w.model, w.seed)
