@@ -244,10 +244,10 @@ def main():
             s_agents, draws, scans, edges, pushed, pt, swept = (int(x) for x in eng.read_counters())
             push_ticks += pt
             # per agent of a tile that ran: state read and write, bitmap bit,
-            # and the pushed count on a push tick; per draw: degree and key;
+            # and the pushed count on a push tick; per draw: degree (1 B) and key;
             # per scan: row_ptr; per scanned entry: index + bitmap; per push:
             # index + atomic read-modify-write
-            touched += (2.125 + (4 if pt else 0)) * swept + 6 * draws + 8 * scans + 5 * edges + 12 * pushed
+            touched += (2.125 + (4 if pt else 0)) * swept + 5 * draws + 8 * scans + 5 * edges + 12 * pushed
         eng.set_counting(False)
         per_launch = b_state / args.ticks
         achieved = per_launch / (launch_us * 1e-6) / 1e9
